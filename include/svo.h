@@ -1,0 +1,371 @@
+/*
+ * svo.h -- C ABI of the MI355X (gfx950) wavefront sparse-voxel-octree cone tracer.
+ *
+ * This is the drop-in boundary for the operator surface of SvoTracer.Kernel
+ * (ComputeManager + KernelName + BufferName, ComputeManager.cs:10-367) and the
+ * buffer/argument layouts of SvoTracer.Domain (the Models directory).  A C# caller binds
+ * these symbols with [DllImport("svo")]; the shim source is in
+ * svo-tracer_amd/csharp/SvoNative.cs.
+ *
+ * Conventions
+ *   - every call returns an int status (SVO_OK == 0, negative on error) and sets
+ *     a thread-local message readable with svo_last_error();
+ *   - sizes are in bytes everywhere (the reference mixes bytes and elements);
+ *   - the library copies host data during the call and never keeps host
+ *     pointers;
+ *   - one context per host thread; all work of a context is serialised on one
+ *     HIP stream (the reference's single in-order queue).
+ *   - struct layouts are the natural OpenCL layouts of wavefront.cl:1-84
+ *     (the reference's host serialisers pack some of them differently).
+ */
+#ifndef SVO_H
+#define SVO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVO_ABI_VERSION 1
+
+/* ---- status codes (HandleResultCode, ComputeManager.cs:283-287, throws on any of these) ---- */
+enum {
+  SVO_OK = 0,
+  SVO_ERR_INVALID_ARGUMENT = -1,
+  SVO_ERR_NO_DEVICE = -2,
+  SVO_ERR_HIP = -3,
+  SVO_ERR_UNKNOWN_BUFFER = -4, /* "Buffer {name} not initializied", ComputeManager.cs:130 */
+  SVO_ERR_UNKNOWN_KERNEL = -5, /* "Kernel name {name} not found", ComputeManager.cs:304,347 */
+  SVO_ERR_UNKNOWN_PARAMETER = -6,
+  SVO_ERR_BUFFER_TOO_SMALL = -7,
+  SVO_ERR_OUT_OF_MEMORY = -8,
+  SVO_ERR_NOT_SUPPORTED = -9,
+  SVO_ERR_CAPACITY = -10,
+  SVO_ERR_IO = -11,
+  SVO_ERR_MISSING_ARGUMENT = -12
+};
+
+/* ---- KernelName ordinals, KernelName.cs:11-21 ---- */
+enum {
+  SVO_KERNEL_PRUNE = 0,
+  SVO_KERNEL_GRAFT = 1,
+  SVO_KERNEL_TRACE = 2,
+  SVO_KERNEL_TEST = 3,
+  SVO_KERNEL_INIT = 4,
+  SVO_KERNEL_RUN_BASE_TRACE = 5,
+  SVO_KERNEL_RUN_BLOCK_TRACE = 6,
+  SVO_KERNEL_EVALUATE_BACKGROUND = 7,
+  SVO_KERNEL_EVALUATE_MATERIAL = 8,
+  SVO_KERNEL_RESOLVE_REMAINDERS = 9,
+  SVO_KERNEL_DRAW_TRACE = 10,
+  SVO_KERNEL_COUNT = 11
+};
+
+/* ---- BufferName ordinals, BufferName.cs:11-63 ---- */
+enum {
+  SVO_BUF_BASES = 0,
+  SVO_BUF_BLOCKS,
+  SVO_BUF_USAGES,
+  SVO_BUF_CHILD_REQUEST_ID,
+  SVO_BUF_CHILD_REQUESTS,
+  SVO_BUF_ORIGINS,
+  SVO_BUF_FOVS,
+  SVO_BUF_DIRECTIONS,
+  SVO_BUF_LOCATIONS,
+  SVO_BUF_DEPTHS,
+  SVO_BUF_WEIGHTINGS,
+  SVO_BUF_PARENT_TRACES,
+  SVO_BUF_ROOT_DIRECTIONS,
+  SVO_BUF_ROOT_LOCATIONS,
+  SVO_BUF_ROOT_DEPTHS,
+  SVO_BUF_ROOT_WEIGHTINGS,
+  SVO_BUF_ROOT_PARENT_TRACES,
+  SVO_BUF_BASE_TRACES,
+  SVO_BUF_BLOCK_TRACES,
+  SVO_BUF_BASE_TRACE_QUEUE,
+  SVO_BUF_BASE_TRACE_QUEUE_ID,
+  SVO_BUF_BLOCK_TRACE_QUEUE,
+  SVO_BUF_BLOCK_TRACE_QUEUE_ID,
+  SVO_BUF_BACKGROUND_QUEUE,
+  SVO_BUF_BACKGROUND_QUEUE_ID,
+  SVO_BUF_MATERIAL_QUEUE,
+  SVO_BUF_MATERIAL_QUEUE_ID,
+  SVO_BUF_COLOUR_RS,
+  SVO_BUF_COLOUR_GS,
+  SVO_BUF_COLOUR_BS,
+  SVO_BUF_LUMINOSITIES,
+  SVO_BUF_RAY_LENGTHS,
+  SVO_BUF_FINAL_COLOUR_RS,
+  SVO_BUF_FINAL_COLOUR_GS,
+  SVO_BUF_FINAL_COLOUR_BS,
+  SVO_BUF_FINAL_WEIGHTINGS,
+  SVO_BUF_ACCUMULATOR_ID,
+  SVO_BUF_PARENT_SIZE,
+  SVO_BUF_PARENT_RESIDENCY,
+  SVO_BUF_PARENTS,
+  SVO_BUF_DEREFERENCE_QUEUE,
+  SVO_BUF_DEREFERENCE_REMAINING,
+  SVO_BUF_SEMAPHOR,
+  SVO_BUF_PRUNING,
+  SVO_BUF_PRUNING_BLOCK_DATA,
+  SVO_BUF_PRUNING_ADDRESSES,
+  SVO_BUF_GRAFTING,
+  SVO_BUF_GRAFTING_BLOCKS,
+  SVO_BUF_GRAFTING_ADDRESSES,
+  SVO_BUF_HOLDING_ADDRESSES,
+  SVO_BUF_ADDRESS_POSITION,
+  SVO_BUF_REFERENCE_COUNT, /* = 51, number of reference buffer names */
+
+  /* Extensions with no BufferName counterpart: per-ray outputs of the frame-level
+   * entry points (the reference renders into a GL image instead, wavefront.cl:1835). */
+  SVO_BUF_OUTPUT_IMAGE = 64, /* float4 RGBA per ray (drawTrace's image) */
+  SVO_BUF_HIT_INDICES = 65,  /* u32 per ray: block address hit, UINT32_MAX = background */
+  SVO_BUF_VISIT_COUNTS = 66, /* u32 per ray: base + block steps taken */
+  SVO_BUF_VISIT_HASHES = 67, /* u64 per ray: FNV-1a-64 of the visit words (depth<<32 | index) */
+  SVO_BUF_RAY_STATUS = 68,   /* u8 per ray: bit0 material, bit1 child requested, bit2 step cap */
+  SVO_BUF_NAME_LIMIT = 80
+};
+
+/* buffer flags for svo_buffer_init (InitBuffer/InitDeviceBuffer/InitReadBuffer/InitWriteBuffer,
+ * ComputeManager.cs:135-215; all map to device memory, the flags are advisory) */
+enum { SVO_BUFFER_READ_WRITE = 0, SVO_BUFFER_READ_ONLY = 1, SVO_BUFFER_WRITE_ONLY = 2 };
+
+/* ---- domain structs (natural layouts) ---- */
+
+/* Block, wavefront.cl:1-12 / Block.cs:7-21: 16 bytes */
+typedef struct {
+  uint32_t child;      /* first of 8 consecutive children, UINT32_MAX = none */
+  uint16_t chunk;      /* per octant k: bit 2k air, bit 2k+1 geometry */
+  int16_t normal_pitch;
+  int16_t normal_yaw;
+  uint8_t colour_r, colour_g, colour_b, opacity;
+  uint8_t spare; /* Specularity in Block.cs */
+  uint8_t gloss;
+} svo_block;
+
+/* Usage, wavefront.cl:14-17: 8 bytes, one per group of 8 blocks */
+typedef struct {
+  uint16_t tick; /* 0 free, 1..65533 last used, 65534 multi-parent, 65535 inviolate */
+  uint16_t pad;
+  uint32_t parent;
+} svo_usage;
+
+/* Parent, wavefront.cl:19-22 */
+typedef struct {
+  uint32_t parent_address;
+  uint32_t next_element;
+} svo_parent;
+
+/* Pruning, wavefront.cl:24-36: 20 bytes */
+typedef struct {
+  uint8_t properties; /* bit0 CullChild, bit1 AlterViolability, bit2 MakeInviolate, bit3 UpdateChunk, bit4 BaseBlock */
+  uint8_t depth;      /* 0xFF: address is a block address; else address indexes pruning addresses (Location) */
+  uint16_t pad0;
+  uint32_t address;
+  uint16_t chunk;
+  uint16_t pad1;
+  uint32_t colour_address;
+  uint32_t child_address;
+} svo_pruning;
+
+/* Grafting, wavefront.cl:38-43: 16 bytes */
+typedef struct {
+  uint32_t graft_data_address;
+  uint32_t graft_total_size;
+  uint8_t depth;
+  uint8_t pad[3];
+  uint32_t graft_address;
+} svo_grafting;
+
+/* ChildRequest, wavefront.cl:45-51: 40 bytes */
+typedef struct {
+  uint32_t address;
+  uint16_t tick;
+  uint8_t depth;
+  uint8_t pad0;
+  uint64_t location[3];
+  uint8_t tree_size;
+  uint8_t pad1[7];
+} svo_child_request;
+
+/* TraceInput, wavefront.cl:53-75 / TraceInput.cs:42-61: 104 bytes */
+typedef struct {
+  float origin[3];
+  float facing[9]; /* row-major; rows: forward, right, up (camera basis) */
+  float fov[2];
+  float dof[2];
+  uint32_t screen_size[2];
+  uint8_t max_opacity;
+  uint8_t base_depth;
+  uint16_t tick;
+  uint32_t max_child_request_id;
+  float fov_multiplier;
+  float fov_constant;
+  float weighting_multiplier;
+  float weighting_constant;
+  uint8_t ambient_light_level;
+  uint8_t pad[7];
+} svo_trace_input;
+
+/* UpdateInputData, wavefront.cl:77-84: 20 bytes */
+typedef struct {
+  uint8_t base_depth;
+  uint8_t pad0;
+  uint16_t tick;
+  uint32_t max_child_request_id;
+  uint32_t memory_size;
+  uint32_t offset;
+  uint32_t graft_size;
+} svo_update_input_data;
+
+/* ---- context: ComputeManagerFactory.Build (ComputeManagerFactory.cs:12-89) / Dispose (ComputeManager.cs:357-367) ---- */
+typedef struct svo_context svo_context;
+
+int svo_device_count(int *count);
+int svo_create(int device_ordinal, svo_context **out);
+int svo_destroy(svo_context *ctx);
+const char *svo_last_error(void);
+int svo_abi_version(void);
+
+/* ---- buffers ---- */
+/* InitBuffer<T> / InitDeviceBuffer / InitReadBuffer / InitWriteBuffer, ComputeManager.cs:135-215.
+ * Re-initialising a name releases the old allocation, as the reference does.
+ * host_data may be NULL (device-only buffer, zero-filled). */
+int svo_buffer_init(svo_context *ctx, int buffer_name, size_t bytes, const void *host_data, uint32_t flags);
+/* Release one named buffer (part of Dispose, ComputeManager.cs:359-360). */
+int svo_buffer_release(svo_context *ctx, int buffer_name);
+int svo_buffer_size(svo_context *ctx, int buffer_name, size_t *bytes);
+/* WriteBuffer<T>, ComputeManager.cs:217-223 (the copy is complete when the call returns). */
+int svo_buffer_write(svo_context *ctx, int buffer_name, size_t offset, const void *src, size_t bytes);
+/* ReadBuffer<T>, ComputeManager.cs:225-230 (blocking). */
+int svo_buffer_read(svo_context *ctx, int buffer_name, size_t offset, void *dst, size_t bytes);
+/* FillBuffer<T>, ComputeManager.cs:258-273 (pattern repeated over [offset, offset+bytes)). */
+int svo_buffer_fill(svo_context *ctx, int buffer_name, const void *pattern, size_t pattern_bytes, size_t offset,
+                    size_t bytes);
+/* ResetIDBuffer, ComputeManager.cs:232-241: read a u32 counter, then set it to 0 (blocking). */
+int svo_reset_id(svo_context *ctx, int buffer_name, uint32_t *count_out);
+/* ZeroIDBuffer, ComputeManager.cs:243-249. */
+int svo_zero_id(svo_context *ctx, int buffer_name);
+/* Swap, ComputeManager.cs:251-256: exchanges the handles; kernel arguments bound to either
+ * name must be re-bound afterwards, as in the reference. */
+int svo_swap(svo_context *ctx, int buffer_a, int buffer_b);
+/* Device address of a named buffer (for zero-copy interop; valid until re-init/release). */
+int svo_buffer_device_pointer(svo_context *ctx, int buffer_name, void **device_ptr);
+
+/* ---- kernels: SetArg / Enqueue / Wait / Flush, ComputeManager.cs:72-120, 277-281 ---- */
+/* SetArg(KernelName, string, BufferName), ComputeManager.cs:94-100; parameter names are those of
+ * the argument tables at ComputeManager.cs:313-344. */
+int svo_set_arg_buffer(svo_context *ctx, int kernel_name, const char *param, int buffer_name);
+/* SetArg<T>(KernelName, string, T / T[]), ComputeManager.cs:80-92 (by-value structs such as
+ * TraceInput / UpdateInputData, size checked against the natural layout). */
+int svo_set_arg_bytes(svo_context *ctx, int kernel_name, const char *param, const void *value, size_t bytes);
+/* Enqueue(KernelName, nuint[] workSize), ComputeManager.cs:72-78: global size only. */
+int svo_enqueue(svo_context *ctx, int kernel_name, uint32_t dims, const size_t *global_size);
+/* Flush, ComputeManager.cs:277-281. */
+int svo_flush(svo_context *ctx);
+/* Wait(CLEvent[]), ComputeManager.cs:109-120: waits for everything enqueued so far. */
+int svo_finish(svo_context *ctx);
+
+/* ---- frame-level entry points (keep MainWindow.RunWavefront's loop on the device) ---- */
+
+/* Options of svo_trace_frame; zero-initialised means defaults. */
+typedef struct {
+  uint32_t flags;          /* SVO_FRAME_* */
+  uint32_t max_tree_depth; /* deepest block level that can be reached (stack sizing), 0 = 24 */
+  uint32_t workgroup_size; /* 0 = tuned default; 64, 128, 256 or 512 */
+  uint32_t refill_lanes;   /* lanes idle before a wave fetches new cones, 0 = default */
+  uint32_t stack_levels;   /* LDS short-stack levels, 0 = from max_tree_depth */
+  uint32_t reserved[3];
+} svo_frame_options;
+
+enum {
+  SVO_FRAME_STATE = 1u << 0,  /* also write final Locations (ulong3, 32-B stride) and Depths */
+  SVO_FRAME_QUEUES = 1u << 1, /* also write BackgroundQueue/MaterialQueue (+ their ID counters) */
+  SVO_FRAME_NO_XCD_SPLIT = 1u << 2
+};
+
+/* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block
+ * trace loop, evaluateBackground, resolveRemainders, drawTrace) in a persistent kernel.
+ * Reads Bases, Blocks; updates Usages ticks (if bound); appends ChildRequests and sets
+ * ChildRequestID (if bound); writes the per-ray extension buffers that are bound.
+ * Queue contents are in ascending ray-id order. */
+int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options);
+
+/* Frame totals of the last svo_trace_frame: base steps, block steps, rays, cones. */
+typedef struct {
+  uint64_t base_steps;
+  uint64_t block_steps;
+  uint64_t rays;
+  uint64_t child_requests;
+} svo_frame_stats;
+int svo_frame_stats_get(svo_context *ctx, svo_frame_stats *stats);
+
+/* ---- timing on the context's stream (hipEvent based) ---- */
+int svo_timer_begin(svo_context *ctx);
+int svo_timer_end(svo_context *ctx, float *elapsed_ms);
+/* When enabled, svo_trace_frame brackets its main kernel with events; the accumulated
+ * kernel time and launch count can be read (and reset) here. */
+int svo_kernel_timing_enable(svo_context *ctx, int enable);
+int svo_kernel_timing_get(svo_context *ctx, double *total_ms, uint64_t *launches, int reset);
+
+/* ---- host-side scene construction: TreeBuilder.BuildTree (TreeBuilder.cs:102-150) ---- */
+
+enum {
+  SVO_GEOMETRY_SPHERES = 0,        /* SphereDefinition list (SphereDefinition.cs), TreeBuilder rules */
+  SVO_GEOMETRY_CUBES = 1,          /* CubeDefinition list, axis-aligned boxes */
+  SVO_GEOMETRY_SPHERE_NOISE = 2,   /* unit sphere SDF + value-noise displacement, world [-s,s]^3 */
+  SVO_GEOMETRY_FBM_HEIGHTFIELD = 3 /* fBm heightfield over a square grid, z up */
+};
+
+typedef struct {
+  uint32_t kind;
+  uint32_t count;       /* SPHERES/CUBES: number of shapes */
+  const float *shapes;  /* SPHERES: (cx,cy,cz,r)*count; CUBES: (minx,miny,minz,maxx,maxy,maxz)*count */
+  uint64_t seed;
+  uint32_t octaves;
+  uint32_t resolution;  /* FBM_HEIGHTFIELD: samples per side */
+  float lacunarity;
+  float gain;
+  float amplitude;      /* SPHERE_NOISE: displacement amplitude */
+  float frequency;      /* base frequency of the first octave */
+  float half_extent;    /* SPHERE_NOISE: octree cube covers world [-s, s]^3 */
+  float z_min, z_max;   /* FBM_HEIGHTFIELD: height range inside the unit cube */
+} svo_geometry;
+
+typedef struct svo_tree svo_tree;
+
+typedef struct {
+  uint32_t base_depth;
+  uint32_t max_depth;
+  uint32_t block_count; /* trimmed exactly as TreeBuilder.cs:145 (including the BuildBlocks count rule) */
+  uint32_t base_count;  /* PowSum(base_depth) */
+  uint32_t group_count; /* ceil(block_count / 8) */
+  uint32_t allocated_groups;
+} svo_tree_info;
+
+int svo_tree_build(const svo_geometry *geometry, uint32_t base_depth, uint32_t max_depth, uint32_t max_size,
+                   svo_tree **out);
+int svo_tree_info_get(const svo_tree *tree, svo_tree_info *info);
+int svo_tree_copy_bases(const svo_tree *tree, uint16_t *dst, size_t count);
+int svo_tree_copy_blocks(const svo_tree *tree, svo_block *dst, size_t count);
+/* Initial usage table (MainWindow.cs:129-141 with its defect D11 resolved): groups of the dense
+ * level (g < 8^(B+1)) are inviolate (65535, parent UINT32_MAX); every other group gets
+ * `tick` and the address of the block that points to it (UINT32_MAX when unreferenced). */
+int svo_tree_initial_usages(const svo_tree *tree, uint16_t tick, svo_usage *dst, size_t count);
+/* .oct persistence (OctreeSerializer.cs:13-45). */
+int svo_tree_save_oct(const svo_tree *tree, const char *path);
+int svo_tree_load_oct(const char *path, svo_tree **out);
+int svo_tree_free(svo_tree *tree);
+/* Uploads Bases, Blocks and (optionally) initial Usages of a tree into the named buffers. */
+int svo_context_load_tree(svo_context *ctx, const svo_tree *tree, int with_usages, uint16_t usage_tick);
+
+/* Separable camera table used for ray generation (replaces native_sin/native_cos of
+ * setupInitialTrace, wavefront.cl:563-595): cos_u/sin_u per column, cos_v/sin_v per row. */
+int svo_camera_lut(const svo_trace_input *input, float *cos_u, float *sin_u, float *cos_v, float *sin_v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVO_H */

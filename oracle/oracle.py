@@ -1,0 +1,219 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "liboracle.so"
+
+U64MAX = (1 << 64) - 1
+U32MAX = (1 << 32) - 1
+
+
+class FrameOut(C.Structure):
+    _fields_ = [
+        ("hit", C.c_void_p),
+        ("visits", C.c_void_p),
+        ("hash", C.c_void_p),
+        ("rgba", C.c_void_p),
+        ("status", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("location", C.c_void_p),
+        ("visit_log", C.c_void_p),
+        ("log_stride", C.c_uint32),
+        ("base_steps", C.c_uint64),
+        ("block_steps", C.c_uint64),
+        ("groups_touched", C.c_uint64),
+    ]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_float_to_ulong.argtypes = [C.c_float]
+        L.orc_float_to_ulong.restype = C.c_uint64
+        L.orc_ulong_to_float.argtypes = [C.c_uint64]
+        L.orc_ulong_to_float.restype = C.c_float
+        L.orc_round_ulong.argtypes = [C.c_uint64, C.c_uint32, C.c_int]
+        L.orc_round_ulong.restype = C.c_uint64
+        L.orc_pow_sum.argtypes = [C.c_uint32]
+        L.orc_pow_sum.restype = C.c_uint32
+        L.orc_chunk_position.argtypes = [C.c_uint32, C.POINTER(C.c_uint64)]
+        L.orc_chunk_position.restype = C.c_uint32
+        L.orc_base_location.argtypes = [C.c_uint32, C.POINTER(C.c_uint64)]
+        L.orc_base_location.restype = C.c_uint32
+        L.orc_compare_positions.argtypes = [C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint8)]
+        L.orc_compare_positions.restype = C.c_uint32
+        L.orc_traverse_chunk.argtypes = [C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint8)]
+        L.orc_traverse_chunk.restype = None
+        L.orc_init_ray.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint64), C.POINTER(C.c_float)]
+        L.orc_init_ray.restype = C.c_int
+        L.orc_camera_lut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_camera_lut.restype = None
+        L.orc_trace_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(FrameOut), C.c_int]
+        L.orc_trace_frame.restype = C.c_int
+        L.orc_trace_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                     C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_trace_rays.restype = C.c_int
+        L.orc_wavefront_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                          C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(C.c_uint32),
+                                          C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(FrameOut)]
+        L.orc_wavefront_frame.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _u64x3(v):
+    return (C.c_uint64 * 3)(*[int(x) & U64MAX for x in v])
+
+
+def float_to_ulong(x: float) -> int:
+    return int(lib().orc_float_to_ulong(x))
+
+
+def ulong_to_float(x: int) -> float:
+    return float(lib().orc_ulong_to_float(x))
+
+
+def round_ulong(value: int, depth: int, up: bool) -> int:
+    return int(lib().orc_round_ulong(value, depth, 1 if up else 0))
+
+
+def pow_sum(depth: int) -> int:
+    return int(lib().orc_pow_sum(depth))
+
+
+def chunk_position(depth: int, loc) -> int:
+    return int(lib().orc_chunk_position(depth, _u64x3(loc)))
+
+
+def base_location(depth: int, loc) -> int:
+    return int(lib().orc_base_location(depth, _u64x3(loc)))
+
+
+def compare_positions(depth: int, prev, loc, sign) -> int:
+    return int(lib().orc_compare_positions(depth, _u64x3(prev), _u64x3(loc), (C.c_uint8 * 3)(*sign)))
+
+
+def traverse_chunk(depth: int, loc, direction, inv, sign):
+    l = _u64x3(loc)
+    lib().orc_traverse_chunk(depth, l, (C.c_float * 3)(*direction), (C.c_float * 3)(*inv), (C.c_uint8 * 3)(*sign))
+    return [int(v) for v in l]
+
+
+def init_ray(trace_input, x: int, y: int):
+    d = (C.c_float * 3)()
+    inv = (C.c_float * 3)()
+    loc = (C.c_uint64 * 3)()
+    fov = C.c_float()
+    hit = lib().orc_init_ray(C.byref(trace_input), x, y, d, inv, loc, C.byref(fov))
+    return bool(hit), list(d), list(inv), [int(v) for v in loc], fov.value
+
+
+def camera_lut(trace_input):
+    w, h = trace_input.screen_size[0], trace_input.screen_size[1]
+    cu, su = np.empty(w, np.float32), np.empty(w, np.float32)
+    cv, sv = np.empty(h, np.float32), np.empty(h, np.float32)
+    lib().orc_camera_lut(C.byref(trace_input), cu.ctypes.data, su.ctypes.data, cv.ctypes.data, sv.ctypes.data)
+    return cu, su, cv, sv
+
+
+def _outputs(n: int, log_stride: int):
+    o = {
+        "hit": np.zeros(n, np.uint32),
+        "visits": np.zeros(n, np.uint32),
+        "hash": np.zeros(n, np.uint64),
+        "rgba": np.zeros((n, 4), np.float32),
+        "status": np.zeros(n, np.uint8),
+        "depth": np.zeros(n, np.uint8),
+        "location": np.zeros((n, 3), np.uint64),
+    }
+    if log_stride:
+        o["visit_log"] = np.zeros((n, log_stride), np.uint64)
+    fo = FrameOut()
+    for k in ("hit", "visits", "hash", "rgba", "status", "depth", "location"):
+        setattr(fo, k, o[k].ctypes.data)
+    fo.visit_log = o["visit_log"].ctypes.data if log_stride else None
+    fo.log_stride = log_stride
+    return o, fo
+
+
+def trace_frame(trace_input, bases: np.ndarray, blocks: np.ndarray, usages: Optional[np.ndarray] = None,
+                request_capacity: int = 0, threads: int = 1, log_stride: int = 0) -> dict:
+    """Per-cone tracing of one primary frame.  `usages` (if given) is updated in place."""
+    n = trace_input.screen_size[0] * trace_input.screen_size[1]
+    o, fo = _outputs(n, log_stride)
+    req = np.zeros(max(request_capacity, 1), dtype=_request_dtype())
+    count = C.c_uint32(0)
+    rc = lib().orc_trace_frame(C.byref(trace_input), bases.ctypes.data, bases.size, blocks.ctypes.data, blocks.size,
+                               _ptr(usages), 0 if usages is None else usages.size, req.ctypes.data, request_capacity,
+                               C.byref(count), C.byref(fo), threads)
+    if rc != 0:
+        raise RuntimeError(f"orc_trace_frame failed: {rc}")
+    o["requests"] = req[: min(count.value, request_capacity)]
+    o["request_count"] = count.value
+    o["base_steps"] = fo.base_steps
+    o["block_steps"] = fo.block_steps
+    o["groups_touched"] = fo.groups_touched
+    return o
+
+
+def trace_rays(trace_input, bases: np.ndarray, blocks: np.ndarray, ray_ids: np.ndarray, threads: int = 1):
+    ids = np.ascontiguousarray(ray_ids, dtype=np.uint32)
+    hit = np.zeros(ids.size, np.uint32)
+    visits = np.zeros(ids.size, np.uint32)
+    rc = lib().orc_trace_rays(C.byref(trace_input), bases.ctypes.data, bases.size, blocks.ctypes.data, blocks.size,
+                              ids.ctypes.data, ids.size, hit.ctypes.data, visits.ctypes.data, threads)
+    if rc != 0:
+        raise RuntimeError(f"orc_trace_rays failed: {rc}")
+    return hit, visits
+
+
+def wavefront_frame(trace_input, bases: np.ndarray, blocks: np.ndarray, usages: Optional[np.ndarray] = None,
+                    request_capacity: int = 0, log_stride: int = 0) -> dict:
+    """Exact emulation of MainWindow.RunWavefront with order-preserving queues."""
+    n = trace_input.screen_size[0] * trace_input.screen_size[1]
+    o, fo = _outputs(n, log_stride)
+    req = np.zeros(max(request_capacity, 1), dtype=_request_dtype())
+    bgq = np.zeros(max(n, 1), np.uint32)
+    matq = np.zeros(max(n, 1), np.uint32)
+    count, nbg, nmat, iters = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+    rc = lib().orc_wavefront_frame(C.byref(trace_input), bases.ctypes.data, bases.size, blocks.ctypes.data, blocks.size,
+                                   _ptr(usages), 0 if usages is None else usages.size, req.ctypes.data, request_capacity,
+                                   C.byref(count), bgq.ctypes.data, C.byref(nbg), matq.ctypes.data, C.byref(nmat),
+                                   C.byref(iters), C.byref(fo))
+    if rc != 0:
+        raise RuntimeError(f"orc_wavefront_frame failed: {rc}")
+    o["requests"] = req[: min(count.value, request_capacity)]
+    o["request_count"] = count.value
+    o["background_queue"] = bgq[: nbg.value]
+    o["material_queue"] = matq[: nmat.value]
+    o["iterations"] = iters.value
+    o["base_steps"] = fo.base_steps
+    o["block_steps"] = fo.block_steps
+    o["groups_touched"] = fo.groups_touched
+    return o
+
+
+def _request_dtype():
+    return np.dtype([("address", "<u4"), ("tick", "<u2"), ("depth", "u1"), ("pad0", "u1"), ("location", "<u8", (3,)),
+                     ("tree_size", "u1"), ("pad1", "u1", (7,))])

@@ -1,0 +1,259 @@
+// svo_trace_frame: one primary frame on the device (MainWindow.RunWavefront,
+// MainWindow.cs:356-426, without any host round-trip per traversal iteration).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "svo_internal.hpp"
+#include "svo_kernels.hpp"
+#include "svo_runtime.hpp"
+
+namespace svo {
+
+namespace {
+
+#define HIP_TRY(expr, what)                          \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+// Returns the named buffer when it is bound and holds `need` bytes, nullptr when it is
+// unbound; an undersized bound buffer is an error.
+int optional_buffer(Context *ctx, int name, size_t need, const char *what, void **ptr) {
+  auto b = ctx->get(name);
+  *ptr = nullptr;
+  if (!b) return SVO_OK;
+  if (b->bytes < need)
+    return fail(SVO_ERR_BUFFER_TOO_SMALL, std::string("svo_trace_frame: buffer ") + what + " holds " +
+                                              std::to_string(b->bytes) + " bytes, needs " + std::to_string(need));
+  *ptr = b->ptr;
+  return SVO_OK;
+}
+
+int required_buffer(Context *ctx, int name, size_t need, const char *what, void **ptr) {
+  if (!ctx->get(name)) return fail(SVO_ERR_UNKNOWN_BUFFER, std::string("svo_trace_frame: buffer ") + what + " not initialised");
+  return optional_buffer(ctx, name, need, what, ptr);
+}
+
+int update_lut(Context *ctx, const svo_trace_input &in) {
+  const uint32_t W = in.screen_size[0], H = in.screen_size[1];
+  if (ctx->lut_valid && ctx->lut_w == W && ctx->lut_h == H && std::memcmp(&ctx->lut_fov0, &in.fov[0], 4) == 0 &&
+      std::memcmp(&ctx->lut_fov1, &in.fov[1], 4) == 0)
+    return SVO_OK;
+  std::vector<float> lut(2 * ((size_t)W + H));
+  camera_lut(in, lut.data(), lut.data() + W, lut.data() + 2 * W, lut.data() + 2 * W + H);
+  int rc = ensure_scratch(ctx->scratch_lut, lut.size() * sizeof(float));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->scratch_lut->ptr, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream),
+          "svo_trace_frame: camera table upload");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "svo_trace_frame: camera table sync");
+  ctx->lut_w = W;
+  ctx->lut_h = H;
+  ctx->lut_fov0 = in.fov[0];
+  ctx->lut_fov1 = in.fov[1];
+  ctx->lut_valid = true;
+  return SVO_OK;
+}
+
+}  // namespace
+
+int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, const svo_frame_options &o, bool *bases_in_lds,
+                  uint32_t *stack_levels, LaunchShape *shape) {
+  int wg = o.workgroup_size ? (int)o.workgroup_size : 256;
+  if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: workgroup_size must be 64, 128, 256 or 512");
+  uint32_t max_depth = o.max_tree_depth ? o.max_tree_depth : (ctx->tree_max_depth ? ctx->tree_max_depth : 24u);
+  if (max_depth > 60) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: max_tree_depth must be <= 60");
+  uint32_t levels = o.stack_levels ? o.stack_levels : (max_depth >= B + 2 ? max_depth - (B + 2) + 1 : 1u);
+  levels = std::max(1u, std::min(levels, 32u));
+  const size_t lds_limit = 64 * 1024;
+  size_t bases_bytes = ((size_t)n_bases * 2 + 3) & ~(size_t)3;
+  *bases_in_lds = bases_bytes <= 32 * 1024;
+  if (!*bases_in_lds) bases_bytes = 0;
+  while (levels > 1 && bases_bytes + (size_t)levels * wg * 4 > lds_limit) levels--;
+  size_t lds = bases_bytes + (size_t)levels * wg * 4;
+  if (lds > lds_limit) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: workgroup too large for LDS");
+  int per_cu = 0;
+  if (trace_occupancy(wg, lds, &per_cu) != 0 || per_cu <= 0)
+    return fail(SVO_ERR_HIP, "svo_trace_frame: occupancy query failed");
+  int grid = per_cu * std::max(1, ctx->cu_count);
+  grid = std::max(8, (grid / 8) * 8);
+  shape->workgroup = wg;
+  shape->grid = grid;
+  shape->lds_bytes = lds;
+  *stack_levels = levels;
+  return SVO_OK;
+}
+
+}  // namespace svo
+
+using namespace svo;
+
+extern "C" int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options) {
+  if (!ctx || !input) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: null argument");
+  svo_frame_options o;
+  std::memset(&o, 0, sizeof o);
+  if (options) o = *options;
+  const svo_trace_input &in = *input;
+  const uint32_t W = in.screen_size[0], H = in.screen_size[1], B = in.base_depth;
+  if (W == 0 || H == 0 || (uint64_t)W * H > (1ull << 31))
+    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: screen size must be non-zero and at most 2^31 cones");
+  if (B < 1 || B > 7) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: base_depth must be 1..7");
+  const size_t N = (size_t)W * H;
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+
+  FrameParams p;
+  std::memset(&p, 0, sizeof p);
+  void *ptr = nullptr;
+  const uint32_t n_bases = pow_sum(B);
+  int rc = required_buffer(ctx, SVO_BUF_BASES, (size_t)n_bases * 2, "Bases", &ptr);
+  if (rc) return rc;
+  p.bases = (const uint16_t *)ptr;
+  p.n_bases = n_bases;
+  rc = required_buffer(ctx, SVO_BUF_BLOCKS, 0, "Blocks", &ptr);
+  if (rc) return rc;
+  p.blocks = (const uint4 *)ptr;
+  const size_t nblk = ctx->get(SVO_BUF_BLOCKS)->bytes / 16;
+  p.n_blocks = (uint32_t)std::min<size_t>(nblk, 0xffffffffu);
+  rc = optional_buffer(ctx, SVO_BUF_USAGES, 0, "Usages", &ptr);
+  if (rc) return rc;
+  p.usage16 = (uint16_t *)ptr;
+  p.n_usages = ptr ? (uint32_t)std::min<size_t>(ctx->get(SVO_BUF_USAGES)->bytes / 8, 0xffffffffu) : 0u;
+
+  svo_child_request *requests = nullptr;
+  uint32_t *request_id = nullptr;
+  uint32_t req_capacity = 0;
+  rc = optional_buffer(ctx, SVO_BUF_CHILD_REQUESTS, 0, "ChildRequests", &ptr);
+  if (rc) return rc;
+  requests = (svo_child_request *)ptr;
+  if (requests)
+    req_capacity = (uint32_t)std::min<size_t>(ctx->get(SVO_BUF_CHILD_REQUESTS)->bytes / sizeof(svo_child_request),
+                                              in.max_child_request_id);
+  rc = optional_buffer(ctx, SVO_BUF_CHILD_REQUEST_ID, 4, "ChildRequestID", &ptr);
+  if (rc) return rc;
+  request_id = (uint32_t *)ptr;
+
+  rc = optional_buffer(ctx, SVO_BUF_OUTPUT_IMAGE, 16 * N, "OutputImage", &ptr);
+  if (rc) return rc;
+  p.rgba = (float4 *)ptr;
+  rc = optional_buffer(ctx, SVO_BUF_HIT_INDICES, 4 * N, "HitIndices", &ptr);
+  if (rc) return rc;
+  p.hit = (uint32_t *)ptr;
+  rc = optional_buffer(ctx, SVO_BUF_VISIT_COUNTS, 4 * N, "VisitCounts", &ptr);
+  if (rc) return rc;
+  p.visits = (uint32_t *)ptr;
+  rc = optional_buffer(ctx, SVO_BUF_VISIT_HASHES, 8 * N, "VisitHashes", &ptr);
+  if (rc) return rc;
+  p.hashes = (uint64_t *)ptr;
+  rc = optional_buffer(ctx, SVO_BUF_RAY_STATUS, N, "RayStatus", &ptr);
+  if (rc) return rc;
+  p.status = (uint8_t *)ptr;
+
+  if (o.flags & SVO_FRAME_STATE) {
+    rc = required_buffer(ctx, SVO_BUF_LOCATIONS, 32 * N, "Locations", &ptr);
+    if (rc) return rc;
+    p.loc_out = (uint64_t *)ptr;
+    rc = required_buffer(ctx, SVO_BUF_DEPTHS, N, "Depths", &ptr);
+    if (rc) return rc;
+    p.depth_out = (uint8_t *)ptr;
+  }
+  uint32_t *bg_queue = nullptr, *mat_queue = nullptr, *bg_count = nullptr, *mat_count = nullptr;
+  if (o.flags & SVO_FRAME_QUEUES) {
+    rc = required_buffer(ctx, SVO_BUF_BACKGROUND_QUEUE, 4 * N, "BackgroundQueue", &ptr);
+    if (rc) return rc;
+    bg_queue = (uint32_t *)ptr;
+    rc = required_buffer(ctx, SVO_BUF_MATERIAL_QUEUE, 4 * N, "MaterialQueue", &ptr);
+    if (rc) return rc;
+    mat_queue = (uint32_t *)ptr;
+    rc = required_buffer(ctx, SVO_BUF_BACKGROUND_QUEUE_ID, 4, "BackgroundQueueID", &ptr);
+    if (rc) return rc;
+    bg_count = (uint32_t *)ptr;
+    rc = required_buffer(ctx, SVO_BUF_MATERIAL_QUEUE_ID, 4, "MaterialQueueID", &ptr);
+    if (rc) return rc;
+    mat_count = (uint32_t *)ptr;
+  }
+  const bool compaction = requests || request_id || (o.flags & SVO_FRAME_QUEUES);
+  if (compaction && !p.status) {
+    rc = ensure_scratch(ctx->scratch_status, N);
+    if (rc) return rc;
+    p.status = (uint8_t *)ctx->scratch_status->ptr;
+  }
+  if (requests || request_id) {
+    rc = ensure_scratch(ctx->scratch_requests, N * sizeof(svo_child_request));
+    if (rc) return rc;
+    p.req_sparse = (svo_child_request *)ctx->scratch_requests->ptr;
+  }
+  rc = ensure_scratch(ctx->scratch_small, kSmallScratchBytes);
+  if (rc) return rc;
+  char *small = (char *)ctx->scratch_small->ptr;
+  p.counters = (uint32_t *)(small + kCountersOffset);
+  p.totals = (unsigned long long *)(small + kTotalsOffset);
+  rc = update_lut(ctx, in);
+  if (rc) return rc;
+  p.lut = (const float *)ctx->scratch_lut->ptr;
+
+  for (int k = 0; k < 9; k++) p.facing[k] = in.facing[k];
+  for (int k = 0; k < 3; k++) p.origin[k] = in.origin[k];
+  p.dof[0] = in.dof[0];
+  p.dof[1] = in.dof[1];
+  p.cone_fov = in.fov[0] / (float)W;  // setupInitialTrace: FoV = FoV[0] / ScreenSize.x
+  p.W = W;
+  p.H = H;
+  p.B = B;
+  p.tick = in.tick;
+  p.tiles_x = (W + 7) / 8;
+  const uint32_t tiles = p.tiles_x * ((H + 7) / 8);
+  for (int r = 0; r <= 8; r++) p.range_begin[r] = (uint32_t)(((uint64_t)tiles * r) / 8);
+  p.refill = o.refill_lanes ? std::min<uint32_t>(o.refill_lanes, 64u) : 32u;
+
+  bool bases_in_lds = false;
+  uint32_t levels = 1;
+  LaunchShape shape;
+  rc = choose_launch(ctx, B, n_bases, o, &bases_in_lds, &levels, &shape);
+  if (rc) return rc;
+  p.bases_in_lds = bases_in_lds ? 1u : 0u;
+  p.stack_levels = levels;
+
+  if (launch_frame_prepare(p.counters, p.totals, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "svo_trace_frame: prepare launch");
+  std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+  if (ctx->kernel_timing) {
+    if (ctx->kernel_events_used == ctx->kernel_events.size()) {
+      std::pair<hipEvent_t, hipEvent_t> pr;
+      HIP_TRY(hipEventCreate(&pr.first), "hipEventCreate");
+      HIP_TRY(hipEventCreate(&pr.second), "hipEventCreate");
+      ctx->kernel_events.push_back(pr);
+    }
+    ev = &ctx->kernel_events[ctx->kernel_events_used++];
+    HIP_TRY(hipEventRecord(ev->first, ctx->stream), "hipEventRecord");
+  }
+  if (launch_trace_frame(p, shape, ctx->stream) != 0) return hip_fail(hipGetLastError(), "svo_trace_frame: trace launch");
+  if (ev) HIP_TRY(hipEventRecord(ev->second, ctx->stream), "hipEventRecord");
+
+  if (compaction) {
+    const uint32_t chunks = compaction_chunks((uint32_t)N);
+    rc = ensure_scratch(ctx->scratch_counts, (size_t)std::max(1u, chunks) * 16);
+    if (rc) return rc;
+    CompactParams c;
+    std::memset(&c, 0, sizeof c);
+    c.status = p.status;
+    c.n = (uint32_t)N;
+    c.counts = (uint32_t *)ctx->scratch_counts->ptr;
+    c.bg_queue = bg_queue;
+    c.mat_queue = mat_queue;
+    c.bg_count = bg_count;
+    c.mat_count = mat_count;
+    c.req_count = request_id;
+    c.req_sparse = p.req_sparse;
+    c.requests = requests;
+    c.req_capacity = req_capacity;
+    c.totals = p.totals;
+    if (launch_compaction(c, ctx->stream) != 0) return hip_fail(hipGetLastError(), "svo_trace_frame: compaction launch");
+  }
+  ctx->last_rays = N;
+  return SVO_OK;
+}

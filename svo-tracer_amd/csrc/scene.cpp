@@ -1,0 +1,715 @@
+// Host-side scene construction: a TreeBuilder-compatible octree builder
+// (TreeBuilder.cs:102-256) over pluggable geometry, .oct persistence
+// (OctreeSerializer.cs:13-45) and the camera table used by ray generation.
+//
+// Allocation order follows TreeBuilder exactly: dense Bases for levels 1..B,
+// dense blocks at level B+2 only under level-(B+1) octants holding both air and
+// geometry, a FIFO of free groups (sorted empty level-(B+1) cells, then groups
+// 8^(B+1) .. maxSize/8-1), children assigned depth-first in x-outer/z-inner
+// order of the dense level, and the final trimmed BlockCount rule of
+// BuildBlocks (TreeBuilder.cs:254).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <thread>
+
+#include "svo_internal.hpp"
+
+namespace svo {
+namespace {
+
+struct Cell {
+  uint32_t level, x, y, z;
+};
+
+// Float bounds of a cell, as BoundingVolume/Location.CoordinateRange compute them
+// (exact for levels <= 24).
+struct Box {
+  float min[3], max[3];
+};
+
+Box cell_box(const Cell &c) {
+  Box b;
+  const float scale = std::ldexp(1.0f, -(int)c.level);
+  const uint32_t v[3] = {c.x, c.y, c.z};
+  for (int a = 0; a < 3; a++) {
+    b.min[a] = (float)v[a] * scale;
+    b.max[a] = (float)(v[a] + 1) * scale;
+  }
+  return b;
+}
+
+// NormalAngle (TreeBuilder.cs:328-334) with the axis convention of the decoder
+// normalVector (wavefront.cl:235-245): n = (cosYaw cosPitch, sinYaw cosPitch, sinPitch).
+void normal_angle(double nx, double ny, double nz, int16_t *pitch, int16_t *yaw) {
+  double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+  if (!(len > 0)) {
+    *pitch = 0;
+    *yaw = 0;
+    return;
+  }
+  nx /= len;
+  ny /= len;
+  nz /= len;
+  double p = std::asin(std::max(-1.0, std::min(1.0, nz))) / M_PI;
+  double y = std::atan2(ny, nx) / M_PI;
+  *pitch = (int16_t)(32767.0 * p);
+  *yaw = (int16_t)(32767.0 * y);
+}
+
+inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Lattice value in [-1, 1) from integer coordinates and a seed (SplitMix64 finaliser).
+inline double lattice(uint64_t seed, int64_t x, int64_t y, int64_t z) {
+  uint64_t h = mix64(seed + 0x9e3779b97f4a7c15ull);
+  h = mix64(h ^ (uint64_t)x * 0xd1b54a32d192ed03ull);
+  h = mix64(h ^ (uint64_t)y * 0xabc98388fb8fac03ull);
+  h = mix64(h ^ (uint64_t)z * 0x8cb92ba72f3d8dd7ull);
+  return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+inline double smooth(double t) { return t * t * (3.0 - 2.0 * t); }
+
+double value_noise3(uint64_t seed, double x, double y, double z) {
+  double fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+  int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
+  double tx = smooth(x - fx), ty = smooth(y - fy), tz = smooth(z - fz);
+  double c[2][2][2];
+  for (int k = 0; k < 2; k++)
+    for (int j = 0; j < 2; j++)
+      for (int i = 0; i < 2; i++) c[k][j][i] = lattice(seed, ix + i, iy + j, iz + k);
+  double r = 0;
+  for (int k = 0; k < 2; k++) {
+    double wy0 = c[k][0][0] + (c[k][0][1] - c[k][0][0]) * tx;
+    double wy1 = c[k][1][0] + (c[k][1][1] - c[k][1][0]) * tx;
+    double wz = wy0 + (wy1 - wy0) * ty;
+    r += (k ? tz : 1.0 - tz) * wz;
+  }
+  return r;
+}
+
+double value_noise2(uint64_t seed, double x, double y) {
+  double fx = std::floor(x), fy = std::floor(y);
+  int64_t ix = (int64_t)fx, iy = (int64_t)fy;
+  double tx = smooth(x - fx), ty = smooth(y - fy);
+  double a = lattice(seed, ix, iy, 0), b = lattice(seed, ix + 1, iy, 0);
+  double c = lattice(seed, ix, iy + 1, 0), d = lattice(seed, ix + 1, iy + 1, 0);
+  double ab = a + (b - a) * tx, cd = c + (d - c) * tx;
+  return ab + (cd - ab) * ty;
+}
+
+// Combined geometry interface (the TreeBuilder.ContainsAir/ContainsGeo/GetNormal/
+// GetColour roles, TreeBuilder.cs:30-66).
+class Geometry {
+ public:
+  virtual ~Geometry() {}
+  virtual void classify(const Cell &c, bool *air, bool *geo) const = 0;
+  virtual void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const = 0;
+};
+
+// SphereDefinition (SphereDefinition.cs) combined by TreeBuilder's rules, in float.
+class Spheres : public Geometry {
+ public:
+  struct S {
+    float o[3], r, lo[3], hi[3];
+  };
+  std::vector<S> s;
+  explicit Spheres(const float *p, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+      S q;
+      for (int a = 0; a < 3; a++) q.o[a] = p[4 * i + a];
+      q.r = p[4 * i + 3];
+      for (int a = 0; a < 3; a++) {
+        q.lo[a] = q.o[a] - q.r;
+        q.hi[a] = q.o[a] + q.r;
+      }
+      s.push_back(q);
+    }
+  }
+  static bool within(const S &q, const Box &b) {
+    for (int a = 0; a < 3; a++)
+      if (!(b.min[a] < q.hi[a] && b.max[a] > q.lo[a])) return false;
+    return true;
+  }
+  static bool geo(const S &q, const Box &b) {
+    float d2 = q.r * q.r;
+    for (int a = 0; a < 3; a++) {
+      if (q.o[a] < b.min[a]) {
+        float t = q.o[a] - b.min[a];
+        d2 -= t * t;
+      } else if (q.o[a] > b.max[a]) {
+        float t = q.o[a] - b.max[a];
+        d2 -= t * t;
+      }
+    }
+    return d2 > 0;
+  }
+  static bool air(const S &q, const Box &b) {
+    for (int k = 0; k < 8; k++) {
+      float x = ((k & 1) ? b.max[0] : b.min[0]) - q.o[0];
+      float y = ((k & 2) ? b.max[1] : b.min[1]) - q.o[1];
+      float z = ((k & 4) ? b.max[2] : b.min[2]) - q.o[2];
+      if (std::sqrt((x * x + y * y) + z * z) > q.r) return true;
+    }
+    return false;
+  }
+  void classify(const Cell &c, bool *a, bool *g) const override {
+    Box b = cell_box(c);
+    bool any_air = true, any_geo = false;
+    for (const S &q : s) {
+      if (!within(q, b)) continue;
+      if (!air(q, b)) any_air = false;
+      if (geo(q, b)) any_geo = true;
+    }
+    *a = any_air;
+    *g = any_geo;
+  }
+  void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
+    Box b = cell_box(c);
+    *pitch = 0;
+    *yaw = 0;
+    rgb[0] = rgb[1] = rgb[2] = 0;
+    bool coloured = false;
+    for (const S &q : s) {
+      if (!within(q, b) || !geo(q, b)) continue;
+      float cx = ((b.max[0] - b.min[0]) / 2) + b.min[0];
+      float cy = ((b.max[1] - b.min[1]) / 2) + b.min[1];
+      float cz = ((b.max[2] - b.min[2]) / 2) + b.min[2];
+      normal_angle(cx - q.o[0], cy - q.o[1], cz - q.o[2], pitch, yaw);
+      if (!coloured) {
+        rgb[0] = 255;  // SphereDefinition.Colour
+        rgb[1] = 0;
+        rgb[2] = 0;
+        coloured = true;
+      }
+    }
+  }
+};
+
+// Axis-aligned boxes (the CubeDefinition role): geometry where the cell overlaps a box,
+// air unless the cell lies inside one.
+class Cubes : public Geometry {
+ public:
+  std::vector<Box> boxes;
+  Cubes(const float *p, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+      Box b;
+      for (int a = 0; a < 3; a++) {
+        b.min[a] = p[6 * i + a];
+        b.max[a] = p[6 * i + 3 + a];
+      }
+      boxes.push_back(b);
+    }
+  }
+  void classify(const Cell &c, bool *a, bool *g) const override {
+    Box v = cell_box(c);
+    bool any_geo = false, inside = false;
+    for (const Box &b : boxes) {
+      bool ov = true, in = true;
+      for (int k = 0; k < 3; k++) {
+        if (!(v.min[k] < b.max[k] && v.max[k] > b.min[k])) ov = false;
+        if (!(v.min[k] >= b.min[k] && v.max[k] <= b.max[k])) in = false;
+      }
+      any_geo |= ov;
+      inside |= in;
+    }
+    *g = any_geo;
+    *a = !inside;
+  }
+  void attributes(const Cell &, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
+    *pitch = 0;
+    *yaw = 0;
+    rgb[0] = rgb[1] = rgb[2] = 255;
+  }
+};
+
+uint8_t channel(uint64_t h, int shift, int base) { return (uint8_t)(base + (int)((h >> shift) & 63u)); }
+
+// Unit-sphere SDF with value-noise fBm displacement in world space [-s, s]^3:
+// d(p) = |p| - (1 + A * fbm(p)).  A cell is classified conservatively from the
+// SDF at its centre and the Lipschitz bound of d.
+class SphereNoise : public Geometry {
+ public:
+  uint64_t seed;
+  uint32_t octaves;
+  double lac, gain, amp, freq, s, lip;
+  SphereNoise(const svo_geometry &g)
+      : seed(g.seed), octaves(g.octaves ? g.octaves : 3), lac(g.lacunarity > 0 ? g.lacunarity : 2.0),
+        gain(g.gain > 0 ? g.gain : 0.5), amp(g.amplitude), freq(g.frequency > 0 ? g.frequency : 4.0),
+        s(g.half_extent > 0 ? g.half_extent : 1.25) {
+    double sum_a = 0, sum_af = 0, a = 1, f = freq;
+    for (uint32_t o = 0; o < octaves; o++) {
+      sum_a += a;
+      sum_af += a * f;
+      a *= gain;
+      f *= lac;
+    }
+    // |d/dt smoothstep| <= 1.5, lattice range 2, three axes.
+    double noise_lip = 3.0 * std::sqrt(3.0) * sum_af / sum_a;
+    lip = 1.0 + std::fabs(amp) * noise_lip;
+  }
+  double fbm(double x, double y, double z) const {
+    double sum = 0, norm = 0, a = 1, f = freq;
+    for (uint32_t o = 0; o < octaves; o++) {
+      sum += a * value_noise3(seed + o, x * f, y * f, z * f);
+      norm += a;
+      a *= gain;
+      f *= lac;
+    }
+    return sum / norm;
+  }
+  double sdf(double x, double y, double z) const {
+    return std::sqrt(x * x + y * y + z * z) - (1.0 + amp * fbm(x, y, z));
+  }
+  void centre(const Cell &c, double p[3], double *half) const {
+    double size = 2.0 * s * std::ldexp(1.0, -(int)c.level);
+    const uint32_t v[3] = {c.x, c.y, c.z};
+    for (int a = 0; a < 3; a++) p[a] = ((double)v[a] + 0.5) * size - s;
+    *half = 0.5 * size;
+  }
+  void classify(const Cell &c, bool *a, bool *g) const override {
+    double p[3], h;
+    centre(c, p, &h);
+    double d = sdf(p[0], p[1], p[2]);
+    double bound = lip * h * std::sqrt(3.0);
+    *a = d > -bound;
+    *g = d < bound;
+  }
+  void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
+    double p[3], h;
+    centre(c, p, &h);
+    double gx = sdf(p[0] + h, p[1], p[2]) - sdf(p[0] - h, p[1], p[2]);
+    double gy = sdf(p[0], p[1] + h, p[2]) - sdf(p[0], p[1] - h, p[2]);
+    double gz = sdf(p[0], p[1], p[2] + h) - sdf(p[0], p[1], p[2] - h);
+    normal_angle(gx, gy, gz, pitch, yaw);
+    uint64_t hs = mix64(seed ^ ((uint64_t)c.level << 58) ^ ((uint64_t)c.x << 38) ^ ((uint64_t)c.y << 19) ^ c.z);
+    rgb[0] = channel(hs, 0, 150);
+    rgb[1] = channel(hs, 8, 90);
+    rgb[2] = channel(hs, 16, 60);
+  }
+};
+
+// fBm heightfield on an R x R grid (R a power of two) covering the unit square;
+// each grid cell is a column of constant height in [z_min, z_max].  Classification
+// is exact through a min/max pyramid: geometry iff the cell's bottom is below the
+// highest column of its footprint, air iff its top is above the lowest column.
+class Heightfield : public Geometry {
+ public:
+  uint32_t res = 0, lres = 0;
+  uint64_t seed;
+  std::vector<std::vector<float>> hmin, hmax;  // per pyramid level L (2^L x 2^L)
+  explicit Heightfield(const svo_geometry &g) : seed(g.seed) {
+    res = g.resolution ? g.resolution : 4096;
+    while ((1u << lres) < res) lres++;
+    res = 1u << lres;
+    uint32_t oct = g.octaves ? g.octaves : 6;
+    double lac = g.lacunarity > 0 ? g.lacunarity : 2.0, gain = g.gain > 0 ? g.gain : 0.5;
+    double freq = g.frequency > 0 ? g.frequency : 4.0;
+    double zlo = g.z_min, zhi = g.z_max;
+    if (!(zhi > zlo)) {
+      zlo = 0.25;
+      zhi = 0.55;
+    }
+    hmin.resize(lres + 1);
+    hmax.resize(lres + 1);
+    std::vector<float> &top = hmin[lres];
+    top.resize((size_t)res * res);
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++) {
+      pool.emplace_back([&, t]() {
+        for (uint32_t j = t; j < res; j += nt)
+          for (uint32_t i = 0; i < res; i++) {
+            double x = (i + 0.5) / res, y = (j + 0.5) / res;
+            double sum = 0, norm = 0, a = 1, f = freq;
+            for (uint32_t o = 0; o < oct; o++) {
+              sum += a * value_noise2(seed + o, x * f, y * f);
+              norm += a;
+              a *= gain;
+              f *= lac;
+            }
+            double v = 0.5 + 0.5 * (sum / norm) * 1.6;
+            v = std::min(1.0, std::max(0.0, v));
+            top[(size_t)j * res + i] = (float)(zlo + (zhi - zlo) * v);
+          }
+      });
+    }
+    for (auto &th : pool) th.join();
+    hmax[lres] = top;
+    for (int L = (int)lres - 1; L >= 0; L--) {
+      uint32_t n = 1u << L;
+      hmin[L].resize((size_t)n * n);
+      hmax[L].resize((size_t)n * n);
+      const auto &mn = hmin[L + 1], &mx = hmax[L + 1];
+      for (uint32_t j = 0; j < n; j++)
+        for (uint32_t i = 0; i < n; i++) {
+          size_t a = (size_t)(2 * j) * (2 * n) + 2 * i, b = a + 1, c = a + 2 * n, d = c + 1;
+          hmin[L][(size_t)j * n + i] = std::min(std::min(mn[a], mn[b]), std::min(mn[c], mn[d]));
+          hmax[L][(size_t)j * n + i] = std::max(std::max(mx[a], mx[b]), std::max(mx[c], mx[d]));
+        }
+    }
+  }
+  void footprint(const Cell &c, float *lo, float *hi) const {
+    if (c.level <= lres) {
+      size_t k = (size_t)c.y * (1u << c.level) + c.x;
+      *lo = hmin[c.level][k];
+      *hi = hmax[c.level][k];
+    } else {
+      uint32_t sh = c.level - lres;
+      size_t k = (size_t)(c.y >> sh) * res + (c.x >> sh);
+      *lo = *hi = hmin[lres][k];
+    }
+  }
+  void classify(const Cell &c, bool *a, bool *g) const override {
+    float lo, hi;
+    footprint(c, &lo, &hi);
+    double z0 = std::ldexp((double)c.z, -(int)c.level), z1 = std::ldexp((double)c.z + 1.0, -(int)c.level);
+    *g = z0 < (double)hi;
+    *a = z1 > (double)lo;
+  }
+  float sample(int64_t i, int64_t j) const {
+    i = std::max<int64_t>(0, std::min<int64_t>(res - 1, i));
+    j = std::max<int64_t>(0, std::min<int64_t>(res - 1, j));
+    return hmin[lres][(size_t)j * res + i];
+  }
+  void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
+    // sample-grid coordinates of the footprint centre and a half-width of >= 1 sample
+    double scale = std::ldexp((double)res, -(int)c.level);
+    double cx = (c.x + 0.5) * scale, cy = (c.y + 0.5) * scale;
+    double hw = std::max(1.0, 0.5 * scale);
+    int64_t x0 = (int64_t)std::floor(cx - hw), x1 = (int64_t)std::floor(cx + hw);
+    int64_t y0 = (int64_t)std::floor(cy - hw), y1 = (int64_t)std::floor(cy + hw);
+    int64_t ci = (int64_t)std::floor(cx), cj = (int64_t)std::floor(cy);
+    double dx = (sample(x1, cj) - sample(x0, cj)) / ((double)(x1 - x0) / res);
+    double dy = (sample(ci, y1) - sample(ci, y0)) / ((double)(y1 - y0) / res);
+    normal_angle(-dx, -dy, 1.0, pitch, yaw);
+    float lo, hi;
+    footprint(c, &lo, &hi);
+    double t = std::min(1.0, std::max(0.0, ((double)hi - 0.25) / 0.3));
+    uint64_t hs = mix64(seed ^ ((uint64_t)c.level << 58) ^ ((uint64_t)c.x << 38) ^ ((uint64_t)c.y << 19) ^ c.z);
+    int jitter = (int)(hs & 15u);
+    rgb[0] = (uint8_t)std::min(255.0, 60 + 150 * t + jitter);
+    rgb[1] = (uint8_t)std::min(255.0, 110 + 100 * t + jitter);
+    rgb[2] = (uint8_t)std::min(255.0, 50 + 190 * t * t + jitter);
+  }
+};
+
+// CanHaveChildren, TreeBuilder.cs:291-302.
+inline bool can_have_children(uint16_t chunk) {
+  for (int k = 0; k < 8; k++)
+    if (((chunk >> (2 * k)) & 3u) == 3u) return true;
+  return false;
+}
+
+// The builder proper (TreeBuilder.BuildTree and helpers).
+class Builder {
+ public:
+  const Geometry &geo;
+  Tree &t;
+  uint32_t B, max_depth;
+  std::vector<uint32_t> empties;
+  size_t empty_next = 0;
+  uint64_t range_next = 0, range_end = 0;
+  bool exhausted = false;
+
+  Builder(const Geometry &g, Tree &tree, uint32_t b, uint32_t md) : geo(g), t(tree), B(b), max_depth(md) {}
+
+  // MakeChunk, TreeBuilder.cs:264-284.
+  uint16_t make_chunk(const Cell &c) const {
+    uint16_t chunk = 0;
+    for (uint32_t k = 0; k < 8; k++) {
+      Cell o{c.level + 1, 2 * c.x + (k & 1u), 2 * c.y + ((k >> 1) & 1u), 2 * c.z + ((k >> 2) & 1u)};
+      bool a, g;
+      geo.classify(o, &a, &g);
+      chunk |= (uint16_t)((a ? 1u : 0u) << (2 * k));
+      chunk |= (uint16_t)((g ? 1u : 0u) << (2 * k + 1));
+    }
+    return chunk;
+  }
+
+  // MakeBlock, TreeBuilder.cs:74-93 (ColourB takes colour[1] and ColourG colour[2], as there).
+  svo_block make_block(const Cell &c) const {
+    svo_block b;
+    std::memset(&b, 0, sizeof b);
+    b.chunk = make_chunk(c);
+    b.child = UINT32_MAX;
+    uint8_t rgb[3];
+    geo.attributes(c, &b.normal_pitch, &b.normal_yaw, rgb);
+    b.colour_r = rgb[0];
+    b.colour_b = rgb[1];
+    b.colour_g = rgb[2];
+    b.opacity = 255;
+    return b;
+  }
+
+  bool any_free() const { return empty_next < empties.size() || range_next < range_end; }
+  uint32_t dequeue() {
+    if (empty_next < empties.size()) return empties[empty_next++];
+    return (uint32_t)range_next++;
+  }
+
+  void ensure(size_t n) {
+    if (t.blocks.size() < n) {
+      if (t.blocks.capacity() < n) t.blocks.reserve(std::max(n, 2 * t.blocks.capacity()));
+      svo_block z;
+      std::memset(&z, 0, sizeof z);
+      t.blocks.resize(n, z);
+    }
+    size_t groups = (n + 7) / 8;
+    if (t.group_parent.size() < groups) {
+      if (t.group_parent.capacity() < groups) t.group_parent.reserve(std::max(groups, 2 * t.group_parent.capacity()));
+      t.group_parent.resize(groups, UINT32_MAX);
+    }
+  }
+
+  uint32_t alloc_group(uint32_t parent_address) {
+    uint32_t g = dequeue();
+    uint32_t a = g << 3;
+    ensure((size_t)a + 8);
+    t.group_parent[g] = parent_address;
+    return a;
+  }
+
+  // BuildBlocks, TreeBuilder.cs:237-256.
+  void build_blocks(uint32_t address, const Cell &parent, uint32_t depth) {
+    for (uint32_t i = 0; i < 8; i++) {
+      Cell c{depth, 2 * parent.x + (i & 1u), 2 * parent.y + ((i >> 1) & 1u), 2 * parent.z + ((i >> 2) & 1u)};
+      svo_block blk = make_block(c);
+      if (depth < max_depth && can_have_children(blk.chunk) && any_free()) {
+        uint32_t na = alloc_group(address + i);
+        blk.child = na;
+        build_blocks(na, c, depth + 1);
+      }
+      t.blocks[address + i] = blk;
+      if (t.block_count < address + i) t.block_count = address + i + 1;
+    }
+  }
+
+  int run(uint64_t max_size) {
+    const uint64_t dense_min = 1ull << (3 * B + 6);
+    if (max_size < dense_min) max_size = dense_min;
+    t.base_depth = B;
+    t.max_depth = max_depth;
+    t.block_count = 0;
+    t.bases.assign(pow_sum(B), 0);
+    // BuildBaseChunks, TreeBuilder.cs:152-172
+    for (uint32_t d = 1; d <= B; d++) {
+      uint32_t n = 1u << d;
+      for (uint32_t z = 0; z < n; z++)
+        for (uint32_t y = 0; y < n; y++)
+          for (uint32_t x = 0; x < n; x++) t.bases[pow_sum(d - 1) + interleave3(x, y, z)] = make_chunk(Cell{d, x, y, z});
+    }
+    // PopulateInitialBlocks, TreeBuilder.cs:180-227
+    const uint32_t bd = B + 2, scan = B + 1, sw = 1u << scan;
+    const uint32_t base_start = pow_sum(B - 1);
+    ensure((size_t)1 << (3 * bd));
+    std::vector<uint8_t> children_needed((size_t)1 << (3 * bd), 0);
+    for (uint32_t z = 0; z < sw; z++)
+      for (uint32_t y = 0; y < sw; y++)
+        for (uint32_t x = 0; x < sw; x++) {
+          uint32_t bp = interleave3(x, y, z);
+          if (((t.bases[base_start + (bp >> 3)] >> ((bp & 7u) * 2)) & 3u) == 3u) {
+            for (uint32_t k = 0; k < 8; k++) {
+              uint32_t x2 = k & 1u, y2 = (k >> 1) & 1u, z2 = (k >> 2) & 1u;
+              uint32_t la = (bp << 3) + interleave3(x2, y2, z2);
+              svo_block blk = make_block(Cell{bd, 2 * x + x2, 2 * y + y2, 2 * z + z2});
+              t.blocks[la] = blk;
+              children_needed[la] = can_have_children(blk.chunk) ? 1 : 0;
+              if (t.block_count < la + 1) t.block_count = la + 1;
+            }
+          } else {
+            empties.push_back(bp);
+          }
+        }
+    std::sort(empties.begin(), empties.end());
+    const uint64_t dense_groups = 1ull << (3 * (B + 1));
+    range_next = dense_groups;
+    range_end = (max_size >> 3) > dense_groups ? (max_size >> 3) : dense_groups;
+    // BuildTree main loop, TreeBuilder.cs:120-142
+    if (max_depth > bd) {
+      uint32_t n = 1u << bd;
+      for (uint32_t x = 0; x < n; x++)
+        for (uint32_t y = 0; y < n; y++)
+          for (uint32_t z = 0; z < n; z++) {
+            uint32_t ia = interleave3(x, y, z);
+            if (!children_needed[ia]) continue;
+            if (!any_free()) return fail(SVO_ERR_CAPACITY, "svo_tree_build: free address queue exhausted");
+            uint32_t a = alloc_group(ia);
+            t.blocks[ia].child = a;
+            build_blocks(a, Cell{bd, x, y, z}, bd + 1);
+          }
+    }
+    // Trim, TreeBuilder.cs:145
+    t.blocks.resize(t.block_count);
+    t.group_parent.resize((t.block_count + 7) / 8, UINT32_MAX);
+    return SVO_OK;
+  }
+};
+
+}  // namespace
+
+void camera_lut(const svo_trace_input &in, float *cos_u, float *sin_u, float *cos_v, float *sin_v) {
+  const float w = (float)in.screen_size[0], h = (float)in.screen_size[1];
+  for (uint32_t x = 0; x < in.screen_size[0]; x++) {
+    float u = in.fov[0] * ((w / 2.0f - (float)x) / w);
+    cos_u[x] = (float)std::cos((double)u);
+    sin_u[x] = (float)std::sin((double)u);
+  }
+  for (uint32_t y = 0; y < in.screen_size[1]; y++) {
+    float v = in.fov[1] * ((h / 2.0f - (float)y) / h);
+    cos_v[y] = (float)std::cos((double)v);
+    sin_v[y] = (float)std::sin((double)v);
+  }
+}
+
+}  // namespace svo
+
+using svo::fail;
+
+extern "C" {
+
+int svo_tree_build(const svo_geometry *geometry, uint32_t base_depth, uint32_t max_depth, uint32_t max_size,
+                   svo_tree **out) {
+  if (!geometry || !out) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: null argument");
+  if (base_depth < 1 || base_depth > 7) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: base_depth must be 1..7");
+  if (max_depth > 20) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: max_depth must be <= 20");
+  std::unique_ptr<svo::Geometry> g;
+  try {
+    switch (geometry->kind) {
+      case SVO_GEOMETRY_SPHERES:
+        if (!geometry->shapes && geometry->count) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: no spheres");
+        g.reset(new svo::Spheres(geometry->shapes, geometry->count));
+        break;
+      case SVO_GEOMETRY_CUBES:
+        if (!geometry->shapes && geometry->count) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: no cubes");
+        g.reset(new svo::Cubes(geometry->shapes, geometry->count));
+        break;
+      case SVO_GEOMETRY_SPHERE_NOISE:
+        g.reset(new svo::SphereNoise(*geometry));
+        break;
+      case SVO_GEOMETRY_FBM_HEIGHTFIELD:
+        g.reset(new svo::Heightfield(*geometry));
+        break;
+      default:
+        return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: unknown geometry kind");
+    }
+    std::unique_ptr<svo_tree> tree(new svo_tree());
+    svo::Builder b(*g, tree->t, base_depth, max_depth);
+    int rc = b.run(max_size);
+    if (rc != SVO_OK) return rc;
+    *out = tree.release();
+    return SVO_OK;
+  } catch (const std::bad_alloc &) {
+    return fail(SVO_ERR_OUT_OF_MEMORY, "svo_tree_build: out of host memory");
+  }
+}
+
+int svo_tree_info_get(const svo_tree *tree, svo_tree_info *info) {
+  if (!tree || !info) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_info_get: null argument");
+  info->base_depth = tree->t.base_depth;
+  info->max_depth = tree->t.max_depth;
+  info->block_count = tree->t.block_count;
+  info->base_count = (uint32_t)tree->t.bases.size();
+  info->group_count = (tree->t.block_count + 7) / 8;
+  uint32_t alloc = 0;
+  for (uint32_t p : tree->t.group_parent) alloc += p != UINT32_MAX;
+  info->allocated_groups = alloc;
+  return SVO_OK;
+}
+
+int svo_tree_copy_bases(const svo_tree *tree, uint16_t *dst, size_t count) {
+  if (!tree || !dst) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_copy_bases: null argument");
+  if (count < tree->t.bases.size()) return fail(SVO_ERR_BUFFER_TOO_SMALL, "svo_tree_copy_bases: destination too small");
+  std::memcpy(dst, tree->t.bases.data(), tree->t.bases.size() * sizeof(uint16_t));
+  return SVO_OK;
+}
+
+int svo_tree_copy_blocks(const svo_tree *tree, svo_block *dst, size_t count) {
+  if (!tree || !dst) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_copy_blocks: null argument");
+  if (count < tree->t.blocks.size()) return fail(SVO_ERR_BUFFER_TOO_SMALL, "svo_tree_copy_blocks: destination too small");
+  std::memcpy(dst, tree->t.blocks.data(), tree->t.blocks.size() * sizeof(svo_block));
+  return SVO_OK;
+}
+
+int svo_tree_initial_usages(const svo_tree *tree, uint16_t tick, svo_usage *dst, size_t count) {
+  if (!tree || !dst) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_initial_usages: null argument");
+  const uint64_t dense_groups = 1ull << (3 * (tree->t.base_depth + 1));
+  for (size_t g = 0; g < count; g++) {
+    uint32_t parent = g < tree->t.group_parent.size() ? tree->t.group_parent[g] : UINT32_MAX;
+    svo_usage u;
+    u.pad = 0;
+    if (g < dense_groups) {
+      u.tick = 65535;
+      u.parent = parent;
+    } else if (parent != UINT32_MAX) {
+      u.tick = tick;
+      u.parent = parent;
+    } else {
+      u.tick = 0;
+      u.parent = UINT32_MAX;
+    }
+    dst[g] = u;
+  }
+  return SVO_OK;
+}
+
+int svo_tree_save_oct(const svo_tree *tree, const char *path) {
+  if (!tree || !path) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_save_oct: null argument");
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(SVO_ERR_IO, std::string("svo_tree_save_oct: cannot open ") + path);
+  uint8_t bd = (uint8_t)tree->t.base_depth;
+  uint32_t bc = tree->t.block_count;
+  bool ok = std::fwrite(&bd, 1, 1, f) == 1 && std::fwrite(&bc, 4, 1, f) == 1 &&
+            std::fwrite(tree->t.bases.data(), 2, tree->t.bases.size(), f) == tree->t.bases.size() &&
+            std::fwrite(tree->t.blocks.data(), 16, tree->t.blocks.size(), f) == tree->t.blocks.size();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? SVO_OK : fail(SVO_ERR_IO, "svo_tree_save_oct: write failed");
+}
+
+int svo_tree_load_oct(const char *path, svo_tree **out) {
+  if (!path || !out) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_load_oct: null argument");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(SVO_ERR_IO, std::string("svo_tree_load_oct: cannot open ") + path);
+  std::unique_ptr<svo_tree> tree(new svo_tree());
+  uint8_t bd = 0;
+  uint32_t bc = 0;
+  bool ok = std::fread(&bd, 1, 1, f) == 1 && std::fread(&bc, 4, 1, f) == 1 && bd >= 1 && bd <= 7;
+  if (ok) {
+    tree->t.base_depth = bd;
+    tree->t.block_count = bc;
+    tree->t.bases.resize(svo::pow_sum(bd));
+    tree->t.blocks.resize(bc);
+    ok = std::fread(tree->t.bases.data(), 2, tree->t.bases.size(), f) == tree->t.bases.size() &&
+         std::fread(tree->t.blocks.data(), 16, bc, f) == bc;
+  }
+  std::fclose(f);
+  if (!ok) return fail(SVO_ERR_IO, "svo_tree_load_oct: truncated or malformed file");
+  // parents are reconstructed from the child links
+  tree->t.group_parent.assign((bc + 7) / 8, UINT32_MAX);
+  for (uint32_t a = 0; a < bc; a++) {
+    uint32_t c = tree->t.blocks[a].child;
+    if (c != UINT32_MAX && (c & 7u) == 0 && (c >> 3) < tree->t.group_parent.size() && tree->t.blocks[a].chunk != 0)
+      tree->t.group_parent[c >> 3] = a;
+  }
+  *out = tree.release();
+  return SVO_OK;
+}
+
+int svo_tree_free(svo_tree *tree) {
+  delete tree;
+  return SVO_OK;
+}
+
+int svo_camera_lut(const svo_trace_input *input, float *cos_u, float *sin_u, float *cos_v, float *sin_v) {
+  if (!input || !cos_u || !sin_u || !cos_v || !sin_v) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_camera_lut: null argument");
+  svo::camera_lut(*input, cos_u, sin_u, cos_v, sin_v);
+  return SVO_OK;
+}
+
+}  // extern "C"

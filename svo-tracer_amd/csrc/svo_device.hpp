@@ -1,0 +1,181 @@
+// Device-side fixed-point and traversal helpers for gfx950.
+//
+// All floating-point code here is compiled with -ffp-contract=off and correctly
+// rounded division/sqrt, so results are bit-identical to the defined semantics of
+// the path (see the oracle's header for the list).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svo {
+namespace dev {
+
+constexpr uint64_t kU64Max = ~0ull;
+constexpr uint32_t kU32Max = 0xffffffffu;
+constexpr uint32_t kStepCap = 1u << 20;     // defined termination bound per cone
+constexpr uint32_t kMaxDepth = 60u;         // deepest level a cone may descend to
+constexpr uint64_t kFnvOffset = 0xcbf29ce484222325ull;
+constexpr uint64_t kFnvPrime = 0x100000001b3ull;
+
+// floatToUlong, wavefront.cl:200-207 (saturating, NaN -> 0 where the reference is undefined).
+__device__ __forceinline__ uint64_t float_to_ulong(float x) {
+  if (x >= 0.999999911f) return kU64Max;
+  float a = fabsf(x) * 18446744073709551616.0f;
+  if (a != a) return 0;
+  if (a >= 18446744073709551616.0f) return kU64Max;
+  return (uint64_t)a;
+}
+
+// ulongToFloat, wavefront.cl:209: exact scaling by 2^-64.
+__device__ __forceinline__ float ulong_to_float(uint64_t x) { return (float)x * 5.42101086242752217e-20f; }
+
+// roundUlong, wavefront.cl:211-221.
+__device__ __forceinline__ uint64_t round_ulong(uint64_t v, uint32_t depth, bool up) {
+  const uint64_t keep = ~(kU64Max >> (depth + 1));
+  if (up) return ((v & keep) + (0x8000000000000000ull >> depth)) - v;
+  const uint64_t out = v - ((v & keep) - 1);
+  return out >= v ? v : out;
+}
+
+// chunkPosition, wavefront.cl:428-430.
+__device__ __forceinline__ uint32_t chunk_position(uint32_t depth, uint64_t x, uint64_t y, uint64_t z) {
+  const uint32_t s = 63u - depth;
+  return (uint32_t)((x >> s) & 1u) | ((uint32_t)((y >> s) & 1u) << 1) | ((uint32_t)((z >> s) & 1u) << 2);
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// baseLocation, wavefront.cl:405-411: Morton code of the top `depth` bits (depth 1..10).
+__device__ __forceinline__ uint32_t morton_top(uint32_t depth, uint64_t x, uint64_t y, uint64_t z) {
+  const uint32_t s = 64u - depth;
+  return spread3((uint32_t)(x >> s)) | (spread3((uint32_t)(y >> s)) << 1) | (spread3((uint32_t)(z >> s)) << 2);
+}
+
+// Offset of level d in Bases: powSum(d-1) = 8 + 64 + ... + 8^(d-1) = (8^d - 1)/7 - 1 (1 <= d <= 10).
+__device__ __forceinline__ uint32_t base_offset(uint32_t d) { return ((1u << (3u * d)) - 1u) / 7u - 1u; }
+
+// comparePositions, wavefront.cl:413-426; signs: bit a set when direction axis a >= 0.
+__device__ __forceinline__ uint32_t compare_positions(uint32_t depth, uint64_t px, uint64_t py, uint64_t pz,
+                                                      uint64_t x, uint64_t y, uint64_t z, uint32_t signs) {
+  for (;;) {
+    const uint32_t p = chunk_position(depth, px, py, pz), n = chunk_position(depth, x, y, z);
+    const uint32_t crossed = ~(p ^ signs) & (n ^ signs) & 7u;
+    if (crossed == 0u || depth == 1u) return depth;
+    depth--;
+  }
+}
+
+// leaving, wavefront.cl:433-437.
+__device__ __forceinline__ bool leaving(uint64_t x, uint64_t y, uint64_t z, uint32_t signs) {
+  const bool sx = signs & 1u, sy = signs & 2u, sz = signs & 4u;
+  return (sx ? x == kU64Max : x == 0) || (sy ? y == kU64Max : y == 0) || (sz ? z == kU64Max : z == 0);
+}
+
+// traverseChunk, wavefront.cl:440-497 (the trailing setConeDepth is evaluated lazily
+// where the cone is consumed).  Written branch-free across the dominant axis.
+__device__ __forceinline__ void traverse_chunk(uint32_t depth, uint64_t &x, uint64_t &y, uint64_t &z, float dx,
+                                               float dy, float dz, float ix, float iy, float iz, uint32_t signs) {
+  const bool sx = signs & 1u, sy = signs & 2u, sz = signs & 4u;
+  const uint64_t d0 = round_ulong(x, depth, sx), d1 = round_ulong(y, depth, sy), d2 = round_ulong(z, depth, sz);
+  const float a0 = fabsf((float)d0 * ix), a1 = fabsf((float)d1 * iy), a2 = fabsf((float)d2 * iz);
+  const int m = (a0 <= a1 && a0 <= a2) ? 0 : ((a1 <= a0 && a1 <= a2) ? 1 : 2);
+  const float inv_m = m == 0 ? ix : (m == 1 ? iy : iz);
+  const float ud = ulong_to_float(m == 0 ? d0 : (m == 1 ? d1 : d2));
+  const uint64_t n0 = m == 0 ? d0 : float_to_ulong((dx * inv_m) * ud);
+  const uint64_t n1 = m == 1 ? d1 : float_to_ulong((dy * inv_m) * ud);
+  const uint64_t n2 = m == 2 ? d2 : float_to_ulong((dz * inv_m) * ud);
+  x = sx ? x + n0 : x - n0;
+  y = sy ? y + n1 : y - n1;
+  z = sz ? z + n2 : z - n2;
+  if (sx && x == 0) x = kU64Max;
+  else if (!sx && x == kU64Max) x = 0;
+  if (sy && y == 0) y = kU64Max;
+  else if (!sy && y == kU64Max) y = 0;
+  if (sz && z == 0) z = kU64Max;
+  else if (!sz && z == kU64Max) z = 0;
+}
+
+// coneSize(fast_length(Origin - loc)), wavefront.cl:387-399, returning the cone itself.
+__device__ __forceinline__ float cone_at(float ox, float oy, float oz, uint64_t x, uint64_t y, uint64_t z, float fov,
+                                         float dof0, float dof1) {
+  const float mx = ox - ulong_to_float(x), my = oy - ulong_to_float(y), mz = oz - ulong_to_float(z);
+  const float m = fabsf(sqrtf((mx * mx + my * my) + mz * mz));
+  const float eye = fov * m;
+  const float f = fabsf(dof0 * (dof1 - m));
+  return eye < f ? f : eye;
+}
+
+// 2^-k (0 <= k <= 126) exactly.
+__device__ __forceinline__ float exp2_neg(uint32_t k) { return __uint_as_float((127u - k) << 23); }
+
+// traceIntoVolume, wavefront.cl:639-1088 (see the oracle for the per-axis statement).
+__device__ __forceinline__ bool trace_into_volume(const float o[3], const float d[3], const float inv[3], float fov,
+                                                  uint64_t &x, uint64_t &y, uint64_t &z) {
+  bool lo[3], hi[3];
+  bool out = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    lo[a] = o[a] < 0.0f;
+    hi[a] = o[a] > 1.0f;
+    out |= lo[a] | hi[a];
+  }
+  if (!out) {
+    x = float_to_ulong(o[0]);
+    y = float_to_ulong(o[1]);
+    z = float_to_ulong(o[2]);
+    return true;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if ((lo[a] | hi[a]) && d[a] == 0.0f) return false;
+    const bool pos = d[a] >= 0.0f;
+    if (lo[a] && !pos) return false;
+    if (hi[a] && pos) return false;
+  }
+  const bool c0 = lo[0] | hi[0], c1 = lo[1] | hi[1], c2 = lo[2] | hi[2];
+  const float m0 = c0 ? fabsf(((lo[0] ? 0.0f : 1.0f) - o[0]) * inv[0]) : 0.0f;
+  const float m1 = c1 ? fabsf(((lo[1] ? 0.0f : 1.0f) - o[1]) * inv[1]) : 0.0f;
+  const float m2 = c2 ? fabsf(((lo[2] ? 0.0f : 1.0f) - o[2]) * inv[2]) : 0.0f;
+  // entry plane: first candidate (x,y,z order, the last candidate excluded) whose slab
+  // distance is >= every other candidate's, else the last candidate
+  const int last = c2 ? 2 : (c1 ? 1 : 0);
+  const bool ok0 = (!c1 || m0 >= m1) && (!c2 || m0 >= m2);
+  const bool ok1 = (!c0 || m1 >= m0) && (!c2 || m1 >= m2);
+  const int pick = (c0 && last != 0 && ok0) ? 0 : ((c1 && last != 1 && ok1) ? 1 : last);
+  const float m = pick == 0 ? m0 : (pick == 1 ? m1 : m2);
+  float lf[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) lf[a] = (a == pick) ? (lo[a] ? 0.0f : 1.0f) : o[a] + d[a] * m;
+  x = float_to_ulong(lf[0]);
+  y = float_to_ulong(lf[1]);
+  z = float_to_ulong(lf[2]);
+  const float eye = fov * m;
+  const float f = fabsf(0.0f * (0.0f - m));
+  const float c = eye < f ? f : eye;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    if (lf[a] < -c || lf[a] > 1.0f + c) return false;
+  return true;
+}
+
+// A 16-byte Block record fetched with one 128-bit load; out-of-range addresses read
+// as an all-zero record (defined; the reference reads past its buffer).
+__device__ __forceinline__ uint4 load_block(const uint4 *__restrict__ blocks, uint32_t n, uint32_t a) {
+  return a < n ? blocks[a] : make_uint4(0u, 0u, 0u, 0u);
+}
+__device__ __forceinline__ uint32_t block_child(uint4 q) { return q.x; }
+__device__ __forceinline__ uint32_t block_chunk(uint4 q) { return q.y & 0xffffu; }
+__device__ __forceinline__ uint32_t block_r(uint4 q) { return (q.z >> 16) & 0xffu; }
+__device__ __forceinline__ uint32_t block_g(uint4 q) { return q.z >> 24; }
+__device__ __forceinline__ uint32_t block_b(uint4 q) { return q.w & 0xffu; }
+
+}  // namespace dev
+}  // namespace svo

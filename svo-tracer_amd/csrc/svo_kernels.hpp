@@ -1,0 +1,71 @@
+// Launch interface of the HIP kernels (plain structs, usable from host code).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "svo.h"
+
+namespace svo {
+
+// Everything one primary frame needs; passed by value as the kernel argument.
+struct FrameParams {
+  // tree
+  const uint16_t *bases;
+  uint32_t n_bases;
+  uint32_t bases_in_lds;
+  const uint4 *blocks;
+  uint32_t n_blocks;
+  uint16_t *usage16;  // svo_usage array viewed as u16: tick of group g at [4g]
+  uint32_t n_usages;
+  // camera
+  const float *lut;  // cos_u[W], sin_u[W], cos_v[H], sin_v[H]
+  float facing[9];
+  float origin[3];
+  float dof[2];
+  float cone_fov;
+  uint32_t W, H, B, tick;
+  // work distribution
+  uint32_t tiles_x;
+  uint32_t range_begin[9];
+  uint32_t *counters;
+  unsigned long long *totals;  // [0] base steps [1] block steps [2] child requests
+  // outputs (any may be null)
+  float4 *rgba;
+  uint32_t *hit;
+  uint32_t *visits;
+  uint64_t *hashes;
+  uint8_t *status;
+  uint8_t *depth_out;
+  uint64_t *loc_out;  // ulong3 with a 32-byte stride
+  svo_child_request *req_sparse;
+  // tuning
+  uint32_t stack_levels;
+  uint32_t refill;
+};
+
+struct LaunchShape {
+  int workgroup;
+  int grid;
+  size_t lds_bytes;
+};
+
+struct CompactParams {
+  const uint8_t *status;
+  uint32_t n;
+  uint32_t *counts;  // 4 per chunk
+  uint32_t *bg_queue, *mat_queue, *bg_count, *mat_count, *req_count;
+  const svo_child_request *req_sparse;
+  svo_child_request *requests;
+  uint32_t req_capacity;
+  unsigned long long *totals;
+};
+
+int trace_occupancy(int wg, size_t lds_bytes, int *blocks_per_cu);
+int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream);
+int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream);
+uint32_t compaction_chunks(uint32_t n);
+int launch_compaction(const CompactParams &c, hipStream_t stream);
+
+}  // namespace svo

@@ -1,0 +1,72 @@
+// Context state of libsvo (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "svo.h"
+
+namespace svo {
+
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  uint32_t flags = 0;
+  ~DeviceBuffer();
+};
+
+// A bound kernel argument: either a buffer allocation (captured at bind time, as
+// clSetKernelArg does) or by-value bytes.
+struct Arg {
+  std::shared_ptr<DeviceBuffer> buffer;
+  int buffer_name = -1;
+  std::vector<uint8_t> bytes;
+};
+
+// Layout of the small per-context scratch block.
+constexpr size_t kCountersOffset = 0;   // 16 x u32 work counters
+constexpr size_t kTotalsOffset = 64;    // 4 x u64 frame totals
+constexpr size_t kIdsOffset = 128;      // 4 x u32 internal queue counters
+constexpr size_t kSmallScratchBytes = 256;
+
+int hip_fail(hipError_t e, const char *what);
+int new_buffer(size_t bytes, std::shared_ptr<DeviceBuffer> *out);
+int ensure_scratch(std::shared_ptr<DeviceBuffer> &b, size_t bytes);
+const std::vector<std::string> &kernel_params(int kernel);
+bool valid_buffer_name(int name);
+
+struct Context {
+  int device = 0;
+  int cu_count = 0;
+  std::string arch;
+  hipStream_t stream = nullptr;
+  std::map<int, std::shared_ptr<DeviceBuffer>> buffers;
+  std::map<int, std::map<std::string, Arg>> args;
+  uint32_t tree_max_depth = 0;
+
+  // scratch owned by the context
+  std::shared_ptr<DeviceBuffer> scratch_status, scratch_requests, scratch_small, scratch_counts, scratch_lut;
+  // camera table cache key
+  uint32_t lut_w = 0, lut_h = 0;
+  float lut_fov0 = 0.f, lut_fov1 = 0.f;
+  bool lut_valid = false;
+
+  uint64_t last_rays = 0;
+
+  // timing
+  hipEvent_t timer_begin = nullptr, timer_end = nullptr;
+  bool kernel_timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> kernel_events;
+  size_t kernel_events_used = 0;
+
+  std::shared_ptr<DeviceBuffer> get(int name) const;
+};
+
+}  // namespace svo
+
+struct svo_context : svo::Context {};

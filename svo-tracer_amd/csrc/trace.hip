@@ -1,0 +1,474 @@
+// Fused primary-frame cone tracer for gfx950 (MI355X).
+//
+// One persistent kernel replaces the reference's host-driven wavefront loop
+// (MainWindow.RunWavefront/RunTraces, MainWindow.cs:356-481) and its kernels
+// init / runBaseTrace / runBlockTrace / evaluateBackground / resolveRemainders /
+// drawTrace (wavefront.cl:1352-1850).  Each lane owns one cone and runs the
+// reference step machine to completion; because a cone's steps depend only on
+// its own state and the read-only tree, the per-cone result equals the
+// wavefront schedule's.
+//
+// MI355X mapping:
+//   - persistent workgroups; idle lanes are refilled from a work counter with one
+//     wave-aggregated atomic (ballot + popcount + mbcnt);
+//   - 8 work ranges (bands of 8x8 pixel tiles), range r served first by the
+//     workgroups of XCD r (blockIdx % 8), then stolen by the others, so each
+//     XCD's L2 sees a spatially coherent part of the image;
+//   - the dense Bases levels live in LDS; block records are fetched with one
+//     128-bit load; instead of re-walking from the dense level every step
+//     (getBlockAddress, wavefront.cl:296-304) each lane keeps a short stack of
+//     block addresses in LDS ([level][lane], conflict-free);
+//   - per-cone outputs are written by cone id; queue order is made canonical
+//     (ascending cone id) by the order-preserving compaction kernels below.
+#include <hip/hip_runtime.h>
+
+#include "svo_device.hpp"
+#include "svo_kernels.hpp"
+
+namespace svo {
+
+using namespace dev;
+
+namespace {
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int WG>
+__global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
+  extern __shared__ uint32_t smem[];
+  uint16_t *s_bases = reinterpret_cast<uint16_t *>(smem);
+  const uint32_t bases_words = p.bases_in_lds ? (p.n_bases + 1u) / 2u : 0u;
+  uint32_t *s_stack = smem + bases_words;
+  if (p.bases_in_lds) {
+    for (uint32_t i = threadIdx.x; i < p.n_bases; i += WG) s_bases[i] = p.bases[i];
+  }
+  __syncthreads();
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t B = p.B, S = p.stack_levels;
+  const uint32_t xcd = blockIdx.x & 7u;
+  const float ox = p.origin[0], oy = p.origin[1], oz = p.origin[2];
+
+  uint32_t range_step = 0;  // wave-uniform
+  bool fetching = true;     // wave-uniform
+
+  bool active = false;
+  uint32_t ray = 0, depth = 0, signs = 0, node_addr = 0, last_group = kU32Max, nvis = 0, nblk = 0;
+  uint64_t lx = 0, ly = 0, lz = 0, hash = kFnvOffset;
+  float dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
+  uint4 node = make_uint4(0u, 0u, 0u, 0u);
+  uint32_t tot_vis = 0, tot_blk = 0;
+
+  auto stack_put = [&](uint32_t level, uint32_t a) {
+    const uint32_t rel = level - (B + 2u);
+    if (rel < S) s_stack[rel * WG + tid] = a;
+  };
+  auto stack_get = [&](uint32_t level) -> uint32_t {
+    const uint32_t rel = level - (B + 2u);
+    if (rel < S) return s_stack[rel * WG + tid];
+    // deeper than the stack: walk down from its last level along the current location
+    uint32_t a = s_stack[(S - 1u) * WG + tid];
+    for (uint32_t L = B + 2u + S - 1u; L < level; L++)
+      a = block_child(load_block(p.blocks, p.n_blocks, a)) + chunk_position(L, lx, ly, lz);
+    return a;
+  };
+
+  // Ends the lane's cone: evaluateBackground / resolveRemainders / drawTrace for a primary
+  // cone (weighting 255, parent = itself), plus the per-cone extension outputs.
+  auto finish = [&](bool material, uint32_t addr, uint4 q, bool child, bool tree_size, bool capped) {
+    float4 c;
+    if (!material) {
+      const float dp = (dx * 0.0f + dy * 0.0f) + dz * -1.0f;
+      uint32_t r, g, b;
+      if (dp > 0.8f) {
+        r = g = b = 255u;
+      } else {
+        r = 135u;
+        g = 206u;
+        b = 235u;
+      }
+      const float fw = 255.0f * 255.0f;
+      c = make_float4(fminf((float)(r * 255u) / fw, 1.0f), fminf((float)(g * 255u) / fw, 1.0f),
+                      fminf((float)(b * 255u) / fw, 1.0f), 1.0f);
+      addr = kU32Max;
+    } else if (addr == kU32Max) {
+      c = make_float4(0.75f, 0.0f, 0.0f, 1.0f);
+    } else {
+      const float fw = 255.0f * 255.0f;
+      c = make_float4(fminf((float)(block_r(q) * 255u) / fw, 1.0f), fminf((float)(block_g(q) * 255u) / fw, 1.0f),
+                      fminf((float)(block_b(q) * 255u) / fw, 1.0f), 1.0f);
+    }
+    if (p.rgba) p.rgba[ray] = c;
+    if (p.hit) p.hit[ray] = addr;
+    if (p.visits) p.visits[ray] = nvis;
+    if (p.hashes) p.hashes[ray] = hash;
+    if (p.status) p.status[ray] = (uint8_t)((material ? 1u : 0u) | (child ? 2u : 0u) | (capped ? 4u : 0u));
+    if (p.depth_out) p.depth_out[ray] = (uint8_t)depth;
+    if (p.loc_out) {
+      p.loc_out[4 * (size_t)ray + 0] = lx;
+      p.loc_out[4 * (size_t)ray + 1] = ly;
+      p.loc_out[4 * (size_t)ray + 2] = lz;
+    }
+    if (child && p.req_sparse) {
+      svo_child_request rq;
+      rq.address = addr;
+      rq.tick = (uint16_t)p.tick;
+      rq.depth = (uint8_t)depth;
+      rq.pad0 = 0;
+      rq.location[0] = lx;
+      rq.location[1] = ly;
+      rq.location[2] = lz;
+      rq.tree_size = tree_size ? 1 : 0;
+      for (int k = 0; k < 7; k++) rq.pad1[k] = 0;
+      p.req_sparse[ray] = rq;
+    }
+    tot_vis += nvis;
+    tot_blk += nblk;
+    active = false;
+  };
+
+  for (;;) {
+    // ---- refill idle lanes with new cones (wave-uniform control flow) ----
+    const uint64_t idle = __ballot(!active);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (fetching && (n_idle >= p.refill || n_idle == 64u)) {
+      const uint32_t r = (xcd + range_step) & 7u;
+      const uint32_t t0 = p.range_begin[r], t1 = p.range_begin[r + 1];
+      const uint32_t limit = (t1 - t0) * 64u;
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&p.counters[r], n_idle);
+      base = __shfl(base, (int)leader);
+      if (base >= limit) {
+        if (++range_step == 8u) fetching = false;
+      } else if (!active) {
+        const uint32_t k = base + lanes_below(idle);
+        if (k < limit) {
+          const uint32_t tile = t0 + (k >> 6), l = k & 63u;
+          const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+          const uint32_t x = tx * 8u + (l & 7u), y = ty * 8u + (l >> 3);
+          if (x < p.W && y < p.H) {
+            // setupInitialTrace, wavefront.cl:563-595 (trig from the camera table)
+            ray = x + y * p.W;
+            const float cu = p.lut[x], su = p.lut[p.W + x];
+            const float cv = p.lut[2u * p.W + y], sv = p.lut[2u * p.W + p.H + y];
+            const float m0 = cu * cv, m1 = (0.0f - su) * cv, m2 = sv;
+            dx = (p.facing[0] * m0 + p.facing[3] * m1) + p.facing[6] * m2;
+            dy = (p.facing[1] * m0 + p.facing[4] * m1) + p.facing[7] * m2;
+            dz = (p.facing[2] * m0 + p.facing[5] * m1) + p.facing[8] * m2;
+            ix = 1.0f / dx;
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+            signs = (dx >= 0.0f ? 1u : 0u) | (dy >= 0.0f ? 2u : 0u) | (dz >= 0.0f ? 4u : 0u);
+            nvis = 0;
+            nblk = 0;
+            hash = kFnvOffset;
+            last_group = kU32Max;
+            const float o[3] = {ox, oy, oz}, d[3] = {dx, dy, dz}, inv[3] = {ix, iy, iz};
+            if (trace_into_volume(o, d, inv, p.cone_fov, lx, ly, lz)) {
+              depth = 1;
+              active = true;
+            } else {
+              depth = 0;
+              lx = ly = lz = 0;
+              finish(false, kU32Max, node, false, false, false);
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0ull) {
+      if (!fetching) break;
+      continue;
+    }
+    if (!active) continue;
+
+    // ---- one step of the reference state machine ----
+    if (nvis >= kStepCap) {
+      depth = 0;
+      finish(false, kU32Max, node, false, false, true);
+      continue;
+    }
+    const uint32_t d = depth;
+    const bool is_block = d > B;
+    uint32_t index, chunk;
+    if (!is_block) {
+      // runBaseTrace, wavefront.cl:1400-1463
+      index = base_offset(d) + morton_top(d, lx, ly, lz);
+      chunk = p.bases_in_lds ? (uint32_t)s_bases[index] : (uint32_t)p.bases[index];
+    } else {
+      // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
+      index = node_addr;
+      chunk = block_chunk(node);
+      const uint32_t g = index >> 3;
+      if (g != last_group) {
+        last_group = g;
+        if (g < p.n_usages) {
+          const uint16_t t = p.usage16[4u * (size_t)g];
+          if (t < 65534u && t != (uint16_t)p.tick) p.usage16[4u * (size_t)g] = (uint16_t)p.tick;
+        }
+      }
+    }
+    const uint32_t oct = chunk_position(d, lx, ly, lz);
+    hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
+    nvis++;
+    nblk += is_block ? 1u : 0u;
+
+    if (((chunk >> (2u * oct)) & 2u) == 0u) {
+      // empty octant: advance to the next cell boundary, pop levels that were left
+      const uint64_t px = lx, py = ly, pz = lz;
+      traverse_chunk(d, lx, ly, lz, dx, dy, dz, ix, iy, iz, signs);
+      if (leaving(lx, ly, lz, signs)) {
+        depth = 0;
+        finish(false, kU32Max, node, false, false, false);
+        continue;
+      }
+      uint32_t nd = compare_positions(d, px, py, pz, lx, ly, lz, signs);
+      if (is_block) {
+        if (nd == B + 1u) {
+          nd = B;
+        } else if (nd > B + 1u && nd < d) {
+          node_addr = stack_get(nd);
+          node = load_block(p.blocks, p.n_blocks, node_addr);
+        }
+      }
+      depth = nd;
+    } else if (!is_block) {
+      if (d < B) {
+        depth = d + 1u;
+      } else {
+        depth = B + 2u;
+        node_addr = morton_top(B + 2u, lx, ly, lz);
+        stack_put(B + 2u, node_addr);
+        node = load_block(p.blocks, p.n_blocks, node_addr);
+      }
+    } else {
+      // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
+      const float cone = cone_at(ox, oy, oz, lx, ly, lz, p.cone_fov, p.dof[0], p.dof[1]);
+      if (cone > exp2_neg(B + 2u)) {
+        depth = B + 2u;
+        const uint32_t a = stack_get(B + 2u);
+        finish(true, a, load_block(p.blocks, p.n_blocks, a), false, false, false);
+      } else if (cone > exp2_neg(d)) {
+        depth = d - 1u;
+        node_addr = stack_get(d - 1u);
+        node = load_block(p.blocks, p.n_blocks, node_addr);
+      } else if (block_child(node) == kU32Max) {
+        // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
+        finish(true, node_addr, node, true, cone <= exp2_neg(d + 1u), false);
+      } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
+        const uint32_t a = block_child(node) + oct;
+        depth = d + 1u;
+        stack_put(d + 1u, a);
+        node_addr = a;
+        node = load_block(p.blocks, p.n_blocks, a);
+      } else {
+        finish(true, node_addr, node, false, false, false);
+      }
+    }
+  }
+
+  // frame totals: one 64-bit atomic per wave
+  uint32_t vb = tot_vis - tot_blk, vk = tot_blk;
+  for (int off = 32; off > 0; off >>= 1) {
+    vb += __shfl_xor(vb, off);
+    vk += __shfl_xor(vk, off);
+  }
+  if (lane == 0) {
+    atomicAdd(&p.totals[0], (unsigned long long)vb);
+    atomicAdd(&p.totals[1], (unsigned long long)vk);
+  }
+}
+
+__global__ void frame_prepare_kernel(uint32_t *counters, unsigned long long *totals) {
+  const uint32_t i = threadIdx.x;
+  if (i < 16u) counters[i] = 0u;
+  if (i < 4u) totals[i] = 0ull;
+}
+
+// ---- order-preserving status compaction (ascending cone id) ----
+
+constexpr uint32_t kCompactThreads = 256;
+constexpr uint32_t kCompactPerThread = 16;
+constexpr uint32_t kCompactChunk = kCompactThreads * kCompactPerThread;
+
+__device__ __forceinline__ void load_status16(const uint8_t *status, uint32_t n, uint32_t base, uint8_t s[16]) {
+  if (base + 16u <= n) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(status + base);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = (base + i < n) ? status[base + i] : 0xffu;
+  }
+}
+
+// category bits of one status byte: bit0 background, bit1 material, bit2 child request
+__device__ __forceinline__ uint32_t categories(uint8_t s) {
+  if (s == 0xffu) return 0u;
+  return ((s & 1u) ? 2u : 1u) | ((s & 2u) ? 4u : 0u);
+}
+
+// Exclusive scan of three counters over the 256 threads of a block.
+__device__ void block_scan3(uint32_t v[3], uint32_t total[3]) {
+  __shared__ uint32_t s_wave[3][kCompactThreads / 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t incl[3] = {v[0], v[1], v[2]};
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(incl[c], off);
+      if (lane >= (uint32_t)off) incl[c] += t;
+    }
+  if (lane == 63u)
+    for (int c = 0; c < 3; c++) s_wave[c][wave] = incl[c];
+  __syncthreads();
+  for (int c = 0; c < 3; c++) {
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < kCompactThreads / 64; w++) {
+      if (w < wave) before += s_wave[c][w];
+      all += s_wave[c][w];
+    }
+    v[c] = before + incl[c] - v[c];
+    total[c] = all;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kCompactThreads) status_count_kernel(const uint8_t *status, uint32_t n,
+                                                                        uint32_t *counts) {
+  const uint32_t base = blockIdx.x * kCompactChunk + threadIdx.x * kCompactPerThread;
+  uint8_t s[16];
+  load_status16(status, n, base, s);
+  uint32_t v[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t c = categories(s[i]);
+    v[0] += c & 1u;
+    v[1] += (c >> 1) & 1u;
+    v[2] += (c >> 2) & 1u;
+  }
+  uint32_t total[3];
+  block_scan3(v, total);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; c++) counts[4u * blockIdx.x + c] = total[c];
+}
+
+// Single-workgroup exclusive scan of the per-chunk counts; writes the queue counters.
+__global__ void __launch_bounds__(kCompactThreads) status_scan_kernel(uint32_t *counts, uint32_t chunks,
+                                                                       uint32_t *bg_count, uint32_t *mat_count,
+                                                                       uint32_t *req_count,
+                                                                       unsigned long long *totals) {
+  uint32_t carry[3] = {0u, 0u, 0u};
+  for (uint32_t start = 0; start < chunks; start += kCompactThreads) {
+    const uint32_t i = start + threadIdx.x;
+    uint32_t v[3] = {0u, 0u, 0u};
+    if (i < chunks)
+      for (int c = 0; c < 3; c++) v[c] = counts[4u * i + c];
+    uint32_t total[3];
+    block_scan3(v, total);
+    if (i < chunks)
+      for (int c = 0; c < 3; c++) counts[4u * i + c] = carry[c] + v[c];
+    for (int c = 0; c < 3; c++) carry[c] += total[c];
+  }
+  if (threadIdx.x == 0) {
+    if (bg_count) *bg_count = carry[0];
+    if (mat_count) *mat_count = carry[1];
+    if (req_count) *req_count = carry[2];
+    totals[2] = carry[2];
+  }
+}
+
+__global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
+    const uint8_t *status, uint32_t n, const uint32_t *offsets, uint32_t *bg_queue, uint32_t *mat_queue,
+    const svo_child_request *req_sparse, svo_child_request *requests, uint32_t req_capacity) {
+  const uint32_t base = blockIdx.x * kCompactChunk + threadIdx.x * kCompactPerThread;
+  uint8_t s[16];
+  load_status16(status, n, base, s);
+  uint32_t v[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t c = categories(s[i]);
+    v[0] += c & 1u;
+    v[1] += (c >> 1) & 1u;
+    v[2] += (c >> 2) & 1u;
+  }
+  uint32_t total[3];
+  block_scan3(v, total);
+  uint32_t pos[3];
+  for (int c = 0; c < 3; c++) pos[c] = offsets[4u * blockIdx.x + c] + v[c];
+  for (int i = 0; i < 16; i++) {
+    const uint32_t c = categories(s[i]);
+    const uint32_t id = base + i;
+    if (c & 1u) {
+      if (bg_queue) bg_queue[pos[0]] = id;
+      pos[0]++;
+    }
+    if (c & 2u) {
+      if (mat_queue) mat_queue[pos[1]] = id;
+      pos[1]++;
+    }
+    if (c & 4u) {
+      if (requests && pos[2] < req_capacity) requests[pos[2]] = req_sparse[id];
+      pos[2]++;
+    }
+  }
+}
+
+template <int WG>
+int launch_trace_wg(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
+  hipLaunchKernelGGL(trace_frame_kernel<WG>, dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int WG>
+int occupancy_wg(size_t lds, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_frame_kernel<WG>, WG, lds) == hipSuccess
+             ? 0
+             : -1;
+}
+
+}  // namespace
+
+int trace_occupancy(int wg, size_t lds_bytes, int *blocks_per_cu) {
+  switch (wg) {
+    case 64: return occupancy_wg<64>(lds_bytes, blocks_per_cu);
+    case 128: return occupancy_wg<128>(lds_bytes, blocks_per_cu);
+    case 256: return occupancy_wg<256>(lds_bytes, blocks_per_cu);
+    case 512: return occupancy_wg<512>(lds_bytes, blocks_per_cu);
+    default: return -1;
+  }
+}
+
+int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream) {
+  hipLaunchKernelGGL(frame_prepare_kernel, dim3(1), dim3(64), 0, stream, counters, totals);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
+  switch (shape.workgroup) {
+    case 64: return launch_trace_wg<64>(p, shape, stream);
+    case 128: return launch_trace_wg<128>(p, shape, stream);
+    case 256: return launch_trace_wg<256>(p, shape, stream);
+    case 512: return launch_trace_wg<512>(p, shape, stream);
+    default: return -1;
+  }
+}
+
+uint32_t compaction_chunks(uint32_t n) { return (n + kCompactChunk - 1u) / kCompactChunk; }
+
+int launch_compaction(const CompactParams &c, hipStream_t stream) {
+  const uint32_t chunks = compaction_chunks(c.n);
+  if (chunks == 0) return 0;
+  hipLaunchKernelGGL(status_count_kernel, dim3(chunks), dim3(kCompactThreads), 0, stream, c.status, c.n, c.counts);
+  hipLaunchKernelGGL(status_scan_kernel, dim3(1), dim3(kCompactThreads), 0, stream, c.counts, chunks, c.bg_count,
+                     c.mat_count, c.req_count, c.totals);
+  hipLaunchKernelGGL(status_scatter_kernel, dim3(chunks), dim3(kCompactThreads), 0, stream, c.status, c.n, c.counts,
+                     c.bg_queue, c.mat_queue, c.req_sparse, c.requests, c.req_capacity);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace svo
