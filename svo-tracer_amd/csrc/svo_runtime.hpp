@@ -40,6 +40,21 @@ int ensure_scratch(std::shared_ptr<DeviceBuffer> &b, size_t bytes);
 const std::vector<std::string> &kernel_params(int kernel);
 bool valid_buffer_name(int name);
 
+// Buffers one fused frame reads and writes (pointer + size; null = not bound).
+struct FrameBuffers {
+  struct B {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+  };
+  B bases, blocks, usages, requests, request_id;
+  B rgba, hit, visits, hashes, status;
+  B locations, depths, bg_queue, mat_queue, bg_count, mat_count;
+};
+
+struct Context;
+int update_camera_table(Context *ctx, const svo_trace_input &in);
+int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb);
+
 struct Context {
   int device = 0;
   int cu_count = 0;
@@ -51,6 +66,8 @@ struct Context {
 
   // scratch owned by the context
   std::shared_ptr<DeviceBuffer> scratch_status, scratch_requests, scratch_small, scratch_counts, scratch_lut;
+  // scratch of the kernel-by-kernel path
+  std::shared_ptr<DeviceBuffer> wave_codes, wave_values, wave_staging, wave_counts;
   // camera table cache key
   uint32_t lut_w = 0, lut_h = 0;
   float lut_fov0 = 0.f, lut_fov1 = 0.f;

@@ -160,8 +160,8 @@ def config2_scene(max_depth: int = 12, resolution: int = 4096) -> SceneSpec:
     g.gain = 0.5
     g.frequency = 4.0
     g.resolution = resolution
-    g.z_min = 0.25
-    g.z_max = 0.55
+    g.z_min = 0.36
+    g.z_max = 0.44
     return SceneSpec(f"fbm-heightfield-d{max_depth}", g, 4, max_depth, 0xFFFFFFFF // 8)
 
 
@@ -171,10 +171,10 @@ def config2_camera(frame: int, width: int = 1920, height: int = 1080, frames: in
     rng = SplitMix64(7)
     theta0 = 2.0 * math.pi * rng.uniform()
     radius = 0.30 + 0.05 * rng.uniform()
-    height_z = 0.68 + 0.04 * rng.uniform()
+    height_z = 0.60 + 0.04 * rng.uniform()
     th = theta0 + 2.0 * math.pi * frame / frames
     eye = (0.5 + radius * math.cos(th), 0.5 + radius * math.sin(th), height_z)
-    target = (0.5 + 0.12 * math.cos(th + 1.3), 0.5 + 0.12 * math.sin(th + 1.3), 0.38)
+    target = (0.5 + 0.12 * math.cos(th + 1.3), 0.5 + 0.12 * math.sin(th + 1.3), 0.40)
     fov_x = math.radians(60.0)
     t = (frame % 65533) + 1 if tick is None else tick
     return trace_input(eye, look_at(eye, target), width, height, fov_x, fov_x * height / width, 4, t)
