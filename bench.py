@@ -84,6 +84,10 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--waves", type=int, default=0, help="register budget of the trace kernel (0 = default, or 6)")
+    ap.add_argument("--chunk", type=int, default=0, help="cones a wave claims per atomic")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -96,6 +100,18 @@ def main() -> int:
     if svo.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+
+    if args.calibrate_copy:
+        B = svo.BufferName
+        n = args.calibrate_copy << 20
+        with svo.ComputeManager(0) as cm:
+            cm.InitDeviceBuffer(B.RootLocations, n)
+            cm.InitDeviceBuffer(B.RootDirections, n)
+            for _ in range(5):
+                cm.CopyBuffer(B.RootDirections, B.RootLocations, n)
+            cm.Wait()
+        print(json.dumps({"calibration_copy_bytes": n, "copies": 5}))
+        return 0
 
     W, H, K, Wm = args.width, args.height, args.steps, args.warmup
     N = W * H
@@ -114,7 +130,8 @@ def main() -> int:
     cm.InitDeviceBuffer(B.OutputImage, N, 16)
     cm.InitDeviceBuffer(B.HitIndices, N, 4)
     cm.InitDeviceBuffer(B.VisitCounts, N, 4)
-    opts = dict(workgroup_size=args.workgroup, refill_lanes=args.refill)
+    opts = dict(workgroup_size=args.workgroup, refill_lanes=args.refill, waves_per_simd=args.waves,
+                work_chunk=args.chunk)
     cams = [svo.scenes.config2_camera(f, W, H, frames=100, tick=f + 1) for f in range(K)]
 
     # untimed pass: exact per-frame work counts for the byte model

@@ -244,6 +244,9 @@ int svo_buffer_read(svo_context *ctx, int buffer_name, size_t offset, void *dst,
 /* FillBuffer<T>, ComputeManager.cs:258-273 (pattern repeated over [offset, offset+bytes)). */
 int svo_buffer_fill(svo_context *ctx, int buffer_name, const void *pattern, size_t pattern_bytes, size_t offset,
                     size_t bytes);
+/* Device-to-device copy between two named buffers (the clEnqueueCopyBuffer role; the reference's
+ * ComputeManager has no counterpart).  Asynchronous on the context's stream. */
+int svo_buffer_copy(svo_context *ctx, int dst_name, size_t dst_offset, int src_name, size_t src_offset, size_t bytes);
 /* ResetIDBuffer, ComputeManager.cs:232-241: read a u32 counter, then set it to 0 (blocking). */
 int svo_reset_id(svo_context *ctx, int buffer_name, uint32_t *count_out);
 /* ZeroIDBuffer, ComputeManager.cs:243-249. */
@@ -275,9 +278,11 @@ typedef struct {
   uint32_t flags;          /* SVO_FRAME_* */
   uint32_t max_tree_depth; /* deepest block level that can be reached (stack sizing), 0 = 24 */
   uint32_t workgroup_size; /* 0 = tuned default; 64, 128, 256 or 512 */
-  uint32_t refill_lanes;   /* lanes idle before a wave fetches new cones, 0 = default */
+  uint32_t refill_lanes;   /* lanes idle before a wave fetches new cones, 0 = 64 (whole wave) */
   uint32_t stack_levels;   /* LDS short-stack levels, 0 = from max_tree_depth */
-  uint32_t reserved[3];
+  uint32_t work_chunk;     /* cones a wave claims per atomic (rounded up to 64), 0 = 64 (one 8x8 tile) */
+  uint32_t waves_per_simd; /* register budget of the trace kernel: 0 (default) or 6 waves per SIMD */
+  uint32_t reserved[1];
 } svo_frame_options;
 
 enum {

@@ -229,7 +229,9 @@ class FrameOptions(C.Structure):
         ("workgroup_size", C.c_uint32),
         ("refill_lanes", C.c_uint32),
         ("stack_levels", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("work_chunk", C.c_uint32),
+        ("waves_per_simd", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -298,6 +300,7 @@ _PROTOS = {
     "svo_buffer_read": [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_size_t],
     "svo_buffer_fill": [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t],
     "svo_reset_id": [C.c_void_p, C.c_int, C.POINTER(C.c_uint32)],
+    "svo_buffer_copy": [C.c_void_p, C.c_int, C.c_size_t, C.c_int, C.c_size_t, C.c_size_t],
     "svo_zero_id": [C.c_void_p, C.c_int],
     "svo_swap": [C.c_void_p, C.c_int, C.c_int],
     "svo_buffer_device_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)],
@@ -440,6 +443,10 @@ class ComputeManager:
             size = self.BufferSize(name) - offset
         _check(lib().svo_buffer_fill(self._h, int(name), p.ctypes.data, p.nbytes, offset, size), f"EnqueueFillBuffer:{BufferName(name).name}")
 
+    def CopyBuffer(self, dst: BufferName, src: BufferName, size: int, dst_offset: int = 0, src_offset: int = 0) -> None:
+        """Device-to-device copy (clEnqueueCopyBuffer role; no ComputeManager counterpart)."""
+        _check(lib().svo_buffer_copy(self._h, int(dst), dst_offset, int(src), src_offset, size), "EnqueueCopyBuffer")
+
     def ResetIDBuffer(self, name: BufferName) -> int:
         v = C.c_uint32(0)
         _check(lib().svo_reset_id(self._h, int(name), C.byref(v)), f"EnqueueReadBuffer:{BufferName(name).name}")
@@ -481,13 +488,16 @@ class ComputeManager:
         _check(lib().svo_context_load_tree(self._h, tree.handle, 1 if with_usages else 0, usage_tick), "LoadTree")
 
     def TraceFrame(self, trace_input: TraceInput, flags: int = 0, max_tree_depth: int = 0, workgroup_size: int = 0,
-                   refill_lanes: int = 0, stack_levels: int = 0) -> None:
+                   refill_lanes: int = 0, stack_levels: int = 0, work_chunk: int = 0,
+                   waves_per_simd: int = 0) -> None:
         o = FrameOptions()
         o.flags = flags
         o.max_tree_depth = max_tree_depth
         o.workgroup_size = workgroup_size
         o.refill_lanes = refill_lanes
         o.stack_levels = stack_levels
+        o.work_chunk = work_chunk
+        o.waves_per_simd = waves_per_simd
         _check(lib().svo_trace_frame(self._h, C.byref(trace_input), C.byref(o)), "TraceFrame")
 
     def FrameStats(self) -> FrameStats:

@@ -64,11 +64,16 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, const svo_frame_op
   while (levels > 1 && bases_bytes + (size_t)levels * wg * 4 > lds_limit) levels--;
   size_t lds = bases_bytes + (size_t)levels * wg * 4;
   if (lds > lds_limit) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup too large for LDS");
+  const int waves = o.waves_per_simd ? (int)o.waves_per_simd : 6;
+  if (waves != 6)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: waves_per_simd must be 0 (default) or 6");
   int per_cu = 0;
-  if (trace_occupancy(wg, lds, &per_cu) != 0 || per_cu <= 0) return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");
+  if (trace_occupancy(wg, waves, lds, &per_cu) != 0 || per_cu <= 0)
+    return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");
   int grid = per_cu * std::max(1, ctx->cu_count);
   grid = std::max(8, (grid / 8) * 8);
   shape->workgroup = wg;
+  shape->waves = waves;
   shape->grid = grid;
   shape->lds_bytes = lds;
   *stack_levels = levels;
@@ -159,6 +164,9 @@ int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_op
   const uint32_t tiles = p.tiles_x * ((H + 7) / 8);
   for (int r = 0; r <= 8; r++) p.range_begin[r] = (uint32_t)(((uint64_t)tiles * r) / 8);
   p.refill = o.refill_lanes ? std::min<uint32_t>(o.refill_lanes, 64u) : 64u;
+  // defaults: a wave takes one 8x8 tile per claim and refills only when all its lanes are
+  // idle -- spatial coherence of the 64 cones matters more than lane utilisation
+  p.chunk = o.work_chunk ? ((std::min(o.work_chunk, 1u << 20) + 63u) / 64u) * 64u : 64u;
 
   bool bases_in_lds = false;
   uint32_t levels = 1;

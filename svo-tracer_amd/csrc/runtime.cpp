@@ -261,6 +261,20 @@ int svo_buffer_fill(svo_context *ctx, int name, const void *pattern, size_t patt
   return SVO_OK;
 }
 
+int svo_buffer_copy(svo_context *ctx, int dst_name, size_t dst_offset, int src_name, size_t src_offset, size_t bytes) {
+  CHECK_CTX(ctx, "svo_buffer_copy");
+  std::shared_ptr<DeviceBuffer> d, s;
+  int rc = checked_range(ctx, dst_name, dst_offset, bytes, "svo_buffer_copy", &d);
+  if (rc) return rc;
+  rc = checked_range(ctx, src_name, src_offset, bytes, "svo_buffer_copy", &s);
+  if (rc) return rc;
+  if (!bytes) return SVO_OK;
+  HIP_TRY(hipMemcpyAsync((char *)d->ptr + dst_offset, (const char *)s->ptr + src_offset, bytes, hipMemcpyDeviceToDevice,
+                         ctx->stream),
+          "svo_buffer_copy");
+  return SVO_OK;
+}
+
 int svo_reset_id(svo_context *ctx, int name, uint32_t *count_out) {
   CHECK_CTX(ctx, "svo_reset_id");
   std::shared_ptr<DeviceBuffer> b;

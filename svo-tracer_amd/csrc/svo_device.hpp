@@ -21,7 +21,7 @@ constexpr uint64_t kFnvPrime = 0x100000001b3ull;
 // floatToUlong, wavefront.cl:200-207 (saturating, NaN -> 0 where the reference is undefined).
 __device__ __forceinline__ uint64_t float_to_ulong(float x) {
   if (x >= 0.999999911f) return kU64Max;
-  float a = fabsf(x) * 18446744073709551616.0f;
+  const float a = fabsf(x) * 18446744073709551616.0f;
   if (a != a) return 0;
   if (a >= 18446744073709551616.0f) return kU64Max;
   return (uint64_t)a;
@@ -30,12 +30,13 @@ __device__ __forceinline__ uint64_t float_to_ulong(float x) {
 // ulongToFloat, wavefront.cl:209: exact scaling by 2^-64.
 __device__ __forceinline__ float ulong_to_float(uint64_t x) { return (float)x * 5.42101086242752217e-20f; }
 
-// roundUlong, wavefront.cl:211-221.
+// roundUlong, wavefront.cl:211-221, rearranged: with low = the bits below the level-(depth+1)
+// cell, rounding up is (cell size - low) = (~v & lowmask) + 1, rounding down is low + 1,
+// except in the first cell where the distance is v itself.
 __device__ __forceinline__ uint64_t round_ulong(uint64_t v, uint32_t depth, bool up) {
-  const uint64_t keep = ~(kU64Max >> (depth + 1));
-  if (up) return ((v & keep) + (0x8000000000000000ull >> depth)) - v;
-  const uint64_t out = v - ((v & keep) - 1);
-  return out >= v ? v : out;
+  const uint64_t lowmask = kU64Max >> (depth + 1);
+  if (!up && v <= lowmask) return v;
+  return ((up ? ~v : v) & lowmask) + 1;
 }
 
 // chunkPosition, wavefront.cl:428-430.
@@ -59,48 +60,87 @@ __device__ __forceinline__ uint32_t morton_top(uint32_t depth, uint64_t x, uint6
   return spread3((uint32_t)(x >> s)) | (spread3((uint32_t)(y >> s)) << 1) | (spread3((uint32_t)(z >> s)) << 2);
 }
 
+__device__ __forceinline__ uint32_t spread3_7(uint32_t v) {  // 7 bits -> every third bit
+  v = (v | (v << 8)) & 0x0000f00fu;
+  v = (v | (v << 4)) & 0x000c30c3u;
+  v = (v | (v << 2)) & 0x00249249u;
+  return v;
+}
+
+// Morton code of the top `depth` bits for base levels (depth 1..7).
+__device__ __forceinline__ uint32_t morton_top_base(uint32_t depth, uint64_t x, uint64_t y, uint64_t z) {
+  const uint32_t s = 64u - depth;
+  return spread3_7((uint32_t)(x >> s)) | (spread3_7((uint32_t)(y >> s)) << 1) | (spread3_7((uint32_t)(z >> s)) << 2);
+}
+
 // Offset of level d in Bases: powSum(d-1) = 8 + 64 + ... + 8^(d-1) = (8^d - 1)/7 - 1 (1 <= d <= 10).
 __device__ __forceinline__ uint32_t base_offset(uint32_t d) { return ((1u << (3u * d)) - 1u) / 7u - 1u; }
 
 // comparePositions, wavefront.cl:413-426; signs: bit a set when direction axis a >= 0.
+// Level L counts as crossed when, on some axis, bit 63-L held the far value of the
+// direction before the step and lost it; the reference pops one level per crossed
+// level, starting at `depth`, and stops at level 1.  Stated in closed form: count the
+// consecutive crossed levels from `depth` upwards.  Levels up to 31 live in the high
+// words of the positions.
+__device__ __forceinline__ uint32_t crossed_hi(uint64_t p, uint64_t n, bool pos) {
+  const uint32_t ph = (uint32_t)(p >> 32), nh = (uint32_t)(n >> 32);
+  return (ph ^ nh) & (pos ? ph : ~ph);
+}
+__device__ __forceinline__ uint64_t crossed_64(uint64_t p, uint64_t n, bool pos) { return (p ^ n) & (pos ? p : ~p); }
+
 __device__ __forceinline__ uint32_t compare_positions(uint32_t depth, uint64_t px, uint64_t py, uint64_t pz,
                                                       uint64_t x, uint64_t y, uint64_t z, uint32_t signs) {
-  for (;;) {
-    const uint32_t p = chunk_position(depth, px, py, pz), n = chunk_position(depth, x, y, z);
-    const uint32_t crossed = ~(p ^ signs) & (n ^ signs) & 7u;
-    if (crossed == 0u || depth == 1u) return depth;
-    depth--;
+  const bool sx = signs & 1u, sy = signs & 2u, sz = signs & 4u;
+  uint32_t k;
+  if (depth <= 31u) {
+    const uint32_t c = crossed_hi(px, x, sx) | crossed_hi(py, y, sy) | crossed_hi(pz, z, sz);
+    const uint32_t t = ~(c >> (31u - depth));  // bit j: level depth-j not crossed
+    k = t ? (uint32_t)__builtin_ctz(t) : 32u;
+  } else {
+    const uint64_t c = crossed_64(px, x, sx) | crossed_64(py, y, sy) | crossed_64(pz, z, sz);
+    k = (uint32_t)__builtin_ctzll(~(c >> (63u - depth)));
   }
+  return depth > k ? depth - k : 1u;
 }
 
-// leaving, wavefront.cl:433-437.
+// leaving, wavefront.cl:433-437: a position sits on the face it is moving towards.
 __device__ __forceinline__ bool leaving(uint64_t x, uint64_t y, uint64_t z, uint32_t signs) {
-  const bool sx = signs & 1u, sy = signs & 2u, sz = signs & 4u;
-  return (sx ? x == kU64Max : x == 0) || (sy ? y == kU64Max : y == 0) || (sz ? z == kU64Max : z == 0);
+  return x == ((signs & 1u) ? kU64Max : 0ull) || y == ((signs & 2u) ? kU64Max : 0ull) ||
+         z == ((signs & 4u) ? kU64Max : 0ull);
+}
+
+// One axis of traverseChunk's move: advance by n in the sign direction; landing exactly
+// on the wrapped-around face (0 moving up, ULONG_MAX moving down) means the cube was
+// left, so the position is pinned to the face it moved towards.
+__device__ __forceinline__ uint64_t advance_axis(uint64_t v, uint64_t n, bool pos) {
+  const uint64_t back = pos ? 0ull : kU64Max;  // the face opposite the motion
+  v = pos ? v + n : v - n;
+  return v == back ? ~back : v;
 }
 
 // traverseChunk, wavefront.cl:440-497 (the trailing setConeDepth is evaluated lazily
-// where the cone is consumed).  Written branch-free across the dominant axis.
+// where the cone is consumed).  Branch-free across the dominant axis m; ulongToFloat(d_m)
+// reuses the conversion already made for the comparison (both are (float)d_m scaled by
+// 2^-64), and only the two minor axes are converted back.
 __device__ __forceinline__ void traverse_chunk(uint32_t depth, uint64_t &x, uint64_t &y, uint64_t &z, float dx,
                                                float dy, float dz, float ix, float iy, float iz, uint32_t signs) {
   const bool sx = signs & 1u, sy = signs & 2u, sz = signs & 4u;
   const uint64_t d0 = round_ulong(x, depth, sx), d1 = round_ulong(y, depth, sy), d2 = round_ulong(z, depth, sz);
-  const float a0 = fabsf((float)d0 * ix), a1 = fabsf((float)d1 * iy), a2 = fabsf((float)d2 * iz);
+  const float f0 = (float)d0, f1 = (float)d1, f2 = (float)d2;
+  const float a0 = fabsf(f0 * ix), a1 = fabsf(f1 * iy), a2 = fabsf(f2 * iz);
   const int m = (a0 <= a1 && a0 <= a2) ? 0 : ((a1 <= a0 && a1 <= a2) ? 1 : 2);
   const float inv_m = m == 0 ? ix : (m == 1 ? iy : iz);
-  const float ud = ulong_to_float(m == 0 ? d0 : (m == 1 ? d1 : d2));
-  const uint64_t n0 = m == 0 ? d0 : float_to_ulong((dx * inv_m) * ud);
-  const uint64_t n1 = m == 1 ? d1 : float_to_ulong((dy * inv_m) * ud);
-  const uint64_t n2 = m == 2 ? d2 : float_to_ulong((dz * inv_m) * ud);
-  x = sx ? x + n0 : x - n0;
-  y = sy ? y + n1 : y - n1;
-  z = sz ? z + n2 : z - n2;
-  if (sx && x == 0) x = kU64Max;
-  else if (!sx && x == kU64Max) x = 0;
-  if (sy && y == 0) y = kU64Max;
-  else if (!sy && y == kU64Max) y = 0;
-  if (sz && z == 0) z = kU64Max;
-  else if (!sz && z == kU64Max) z = 0;
+  const float ud = (m == 0 ? f0 : (m == 1 ? f1 : f2)) * 5.42101086242752217e-20f;
+  const uint64_t dm = m == 0 ? d0 : (m == 1 ? d1 : d2);
+  // minor axes: first = (m == 0 ? y : x), second = (m == 2 ? y : z)
+  const uint64_t na = float_to_ulong(((m == 0 ? dy : dx) * inv_m) * ud);
+  const uint64_t nb = float_to_ulong(((m == 2 ? dy : dz) * inv_m) * ud);
+  const uint64_t n0 = m == 0 ? dm : na;
+  const uint64_t n1 = m == 1 ? dm : (m == 0 ? na : nb);
+  const uint64_t n2 = m == 2 ? dm : nb;
+  x = advance_axis(x, n0, sx);
+  y = advance_axis(y, n1, sy);
+  z = advance_axis(z, n2, sz);
 }
 
 // coneSize(fast_length(Origin - loc)), wavefront.cl:387-399, returning the cone itself.

@@ -9,6 +9,9 @@
 
 namespace svo {
 
+// work counters sit on separate 128-byte lines (u32 stride)
+constexpr uint32_t kCounterStride = 32;
+
 // Everything one primary frame needs; passed by value as the kernel argument.
 struct FrameParams {
   // tree
@@ -43,10 +46,12 @@ struct FrameParams {
   // tuning
   uint32_t stack_levels;
   uint32_t refill;
+  uint32_t chunk;  // work items a wave claims per atomic (multiple of 64)
 };
 
 struct LaunchShape {
   int workgroup;
+  int waves;  // register budget: waves per SIMD the kernel is compiled for (6)
   int grid;
   size_t lds_bytes;
 };
@@ -62,7 +67,7 @@ struct CompactParams {
   unsigned long long *totals;
 };
 
-int trace_occupancy(int wg, size_t lds_bytes, int *blocks_per_cu);
+int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu);
 int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream);
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream);
 uint32_t compaction_chunks(uint32_t n);

@@ -29,10 +29,9 @@ struct Arg {
 };
 
 // Layout of the small per-context scratch block.
-constexpr size_t kCountersOffset = 0;   // 16 x u32 work counters
-constexpr size_t kTotalsOffset = 64;    // 4 x u64 frame totals
-constexpr size_t kIdsOffset = 128;      // 4 x u32 internal queue counters
-constexpr size_t kSmallScratchBytes = 256;
+constexpr size_t kCountersOffset = 0;    // 8 work counters, 128 bytes apart
+constexpr size_t kTotalsOffset = 1024;   // 4 x u64 frame totals
+constexpr size_t kSmallScratchBytes = 2048;
 
 int hip_fail(hipError_t e, const char *what);
 int new_buffer(size_t bytes, std::shared_ptr<DeviceBuffer> *out);

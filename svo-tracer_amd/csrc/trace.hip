@@ -35,8 +35,11 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-template <int WG>
-__global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
+// WAVES > 0 asks the compiler for that many waves per SIMD (a register budget).
+// HASH: the visit-sequence hash is computed only when its output is bound.
+template <int WG, int WAVES, bool HASH>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+trace_frame_kernel(const FrameParams p) {
   extern __shared__ uint32_t smem[];
   uint16_t *s_bases = reinterpret_cast<uint16_t *>(smem);
   const uint32_t bases_words = p.bases_in_lds ? (p.n_bases + 1u) / 2u : 0u;
@@ -51,8 +54,9 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
   const uint32_t xcd = blockIdx.x & 7u;
   const float ox = p.origin[0], oy = p.origin[1], oz = p.origin[2];
 
-  uint32_t range_step = 0;  // wave-uniform
+  uint32_t range_step = 0;  // wave-uniform: ranges tried, in order xcd, xcd+1, ...
   bool fetching = true;     // wave-uniform
+  uint32_t cur = 0, cur_end = 0, cur_t0 = 0;  // wave-uniform: the wave's claimed slice of work
 
   bool active = false;
   uint32_t ray = 0, depth = 0, signs = 0, node_addr = 0, last_group = kU32Max, nvis = 0, nblk = 0;
@@ -60,6 +64,14 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
   float dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
   uint4 node = make_uint4(0u, 0u, 0u, 0u);
   uint32_t tot_vis = 0, tot_blk = 0;
+  // updateUsage is resolved one step late so its load never stalls the step; within a
+  // frame every write stores the frame tick, so the deferral does not change the result
+  uint32_t pend_g = kU32Max, pend_t = 0;
+  auto resolve_usage = [&]() {
+    if (pend_g != kU32Max && pend_t < 65534u && pend_t != (p.tick & 0xffffu))
+      p.usage16[4u * (size_t)pend_g] = (uint16_t)p.tick;
+    pend_g = kU32Max;
+  };
 
   auto stack_put = [&](uint32_t level, uint32_t a) {
     const uint32_t rel = level - (B + 2u);
@@ -103,7 +115,7 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
     if (p.rgba) p.rgba[ray] = c;
     if (p.hit) p.hit[ray] = addr;
     if (p.visits) p.visits[ray] = nvis;
-    if (p.hashes) p.hashes[ray] = hash;
+    if (HASH) p.hashes[ray] = hash;
     if (p.status) p.status[ray] = (uint8_t)((material ? 1u : 0u) | (child ? 2u : 0u) | (capped ? 4u : 0u));
     if (p.depth_out) p.depth_out[ray] = (uint8_t)depth;
     if (p.loc_out) {
@@ -124,6 +136,7 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
       for (int k = 0; k < 7; k++) rq.pad1[k] = 0;
       p.req_sparse[ray] = rq;
     }
+    resolve_usage();
     tot_vis += nvis;
     tot_blk += nblk;
     active = false;
@@ -134,19 +147,29 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (fetching && (n_idle >= p.refill || n_idle == 64u)) {
-      const uint32_t r = (xcd + range_step) & 7u;
-      const uint32_t t0 = p.range_begin[r], t1 = p.range_begin[r + 1];
-      const uint32_t limit = (t1 - t0) * 64u;
-      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(&p.counters[r], n_idle);
-      base = __shfl(base, (int)leader);
-      if (base >= limit) {
-        if (++range_step == 8u) fetching = false;
-      } else if (!active) {
-        const uint32_t k = base + lanes_below(idle);
-        if (k < limit) {
-          const uint32_t tile = t0 + (k >> 6), l = k & 63u;
+      // claim a slice of `chunk` work items with one atomic when the wave's slice is used up
+      while (cur >= cur_end && range_step < 8u) {
+        const uint32_t r = (xcd + range_step) & 7u;
+        const uint32_t limit = (p.range_begin[r + 1] - p.range_begin[r]) * 64u;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&p.counters[r * kCounterStride], p.chunk);
+        base = __shfl(base, 0);
+        if (base < limit) {
+          cur = base;
+          cur_end = min(base + p.chunk, limit);
+          cur_t0 = p.range_begin[r];
+        } else {
+          range_step++;
+        }
+      }
+      if (cur >= cur_end) {
+        fetching = false;
+      } else {
+        const uint32_t k = cur + lanes_below(idle);
+        const uint32_t take = min(n_idle, cur_end - cur);
+        cur += take;
+        if (!active && k < cur_end) {
+          const uint32_t tile = cur_t0 + (k >> 6), l = k & 63u;
           const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
           const uint32_t x = tx * 8u + (l & 7u), y = ty * 8u + (l >> 3);
           if (x < p.W && y < p.H) {
@@ -196,7 +219,7 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
     uint32_t index, chunk;
     if (!is_block) {
       // runBaseTrace, wavefront.cl:1400-1463
-      index = base_offset(d) + morton_top(d, lx, ly, lz);
+      index = base_offset(d) + (B <= 7u ? morton_top_base(d, lx, ly, lz) : morton_top(d, lx, ly, lz));
       chunk = p.bases_in_lds ? (uint32_t)s_bases[index] : (uint32_t)p.bases[index];
     } else {
       // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
@@ -205,14 +228,15 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
       const uint32_t g = index >> 3;
       if (g != last_group) {
         last_group = g;
+        resolve_usage();
         if (g < p.n_usages) {
-          const uint16_t t = p.usage16[4u * (size_t)g];
-          if (t < 65534u && t != (uint16_t)p.tick) p.usage16[4u * (size_t)g] = (uint16_t)p.tick;
+          pend_g = g;
+          pend_t = p.usage16[4u * (size_t)g];
         }
       }
     }
     const uint32_t oct = chunk_position(d, lx, ly, lz);
-    hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
+    if (HASH) hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
     nvis++;
     nblk += is_block ? 1u : 0u;
 
@@ -284,7 +308,7 @@ __global__ void __launch_bounds__(WG) trace_frame_kernel(const FrameParams p) {
 
 __global__ void frame_prepare_kernel(uint32_t *counters, unsigned long long *totals) {
   const uint32_t i = threadIdx.x;
-  if (i < 16u) counters[i] = 0u;
+  if (i < 8u) counters[i * kCounterStride] = 0u;
   if (i < 4u) totals[i] = 0ull;
 }
 
@@ -418,27 +442,52 @@ __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
   }
 }
 
-template <int WG>
+template <int WG, int WAVES>
 int launch_trace_wg(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
-  hipLaunchKernelGGL(trace_frame_kernel<WG>, dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
+  if (p.hashes)
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
+  else
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int WG>
+// Occupancy of the hashing variant: it never holds fewer registers than the plain one,
+// so a grid sized by it fits both.
+template <int WG, int WAVES>
 int occupancy_wg(size_t lds, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_frame_kernel<WG>, WG, lds) == hipSuccess
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_frame_kernel<WG, WAVES, true>, WG, lds) ==
+                 hipSuccess
              ? 0
              : -1;
 }
 
+template <int WAVES>
+int launch_waves(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
+  switch (shape.workgroup) {
+    case 64: return launch_trace_wg<64, WAVES>(p, shape, stream);
+    case 128: return launch_trace_wg<128, WAVES>(p, shape, stream);
+    case 256: return launch_trace_wg<256, WAVES>(p, shape, stream);
+    case 512: return launch_trace_wg<512, WAVES>(p, shape, stream);
+    default: return -1;
+  }
+}
+
+template <int WAVES>
+int occupancy_waves(int wg, size_t lds, int *blocks_per_cu) {
+  switch (wg) {
+    case 64: return occupancy_wg<64, WAVES>(lds, blocks_per_cu);
+    case 128: return occupancy_wg<128, WAVES>(lds, blocks_per_cu);
+    case 256: return occupancy_wg<256, WAVES>(lds, blocks_per_cu);
+    case 512: return occupancy_wg<512, WAVES>(lds, blocks_per_cu);
+    default: return -1;
+  }
+}
+
 }  // namespace
 
-int trace_occupancy(int wg, size_t lds_bytes, int *blocks_per_cu) {
-  switch (wg) {
-    case 64: return occupancy_wg<64>(lds_bytes, blocks_per_cu);
-    case 128: return occupancy_wg<128>(lds_bytes, blocks_per_cu);
-    case 256: return occupancy_wg<256>(lds_bytes, blocks_per_cu);
-    case 512: return occupancy_wg<512>(lds_bytes, blocks_per_cu);
+int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
+  switch (waves) {
+    case 6: return occupancy_waves<6>(wg, lds_bytes, blocks_per_cu);
     default: return -1;
   }
 }
@@ -449,11 +498,8 @@ int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStre
 }
 
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
-  switch (shape.workgroup) {
-    case 64: return launch_trace_wg<64>(p, shape, stream);
-    case 128: return launch_trace_wg<128>(p, shape, stream);
-    case 256: return launch_trace_wg<256>(p, shape, stream);
-    case 512: return launch_trace_wg<512>(p, shape, stream);
+  switch (shape.waves) {
+    case 6: return launch_waves<6>(p, shape, stream);
     default: return -1;
   }
 }
