@@ -35,6 +35,22 @@ class FrameOut(C.Structure):
     ]
 
 
+class Pool(C.Structure):
+    _fields_ = [
+        ("bases", C.c_void_p), ("n_bases", C.c_uint32),
+        ("blocks", C.c_void_p), ("n_blocks", C.c_uint32),
+        ("usages", C.c_void_p), ("n_usages", C.c_uint32),
+        ("parents", C.c_void_p), ("n_parents", C.c_uint32),
+        ("residency", C.c_void_p),
+        ("requests", C.c_void_p), ("request_capacity", C.c_uint32),
+        ("request_count", C.POINTER(C.c_uint32)),
+    ]
+
+
+class TreeStats(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in ("applied", "skipped", "invalid", "requests", "collected", "placed", "unlinks")]
+
+
 _lib: Optional[C.CDLL] = None
 
 
@@ -74,6 +90,16 @@ def lib() -> C.CDLL:
                                           C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(C.c_uint32),
                                           C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(FrameOut)]
         L.orc_wavefront_frame.restype = C.c_int
+        L.orc_prune.argtypes = [C.POINTER(Pool), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                C.c_uint32, C.POINTER(TreeStats)]
+        L.orc_prune.restype = C.c_int
+        L.orc_collect.argtypes = [C.POINTER(Pool), C.c_void_p, C.c_uint32, C.c_void_p]
+        L.orc_collect.restype = C.c_uint32
+        L.orc_graft.argtypes = [C.POINTER(Pool), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                C.c_uint32, C.POINTER(TreeStats)]
+        L.orc_graft.restype = C.c_int
+        L.orc_tick_age.argtypes = [C.c_uint16, C.c_uint16]
+        L.orc_tick_age.restype = C.c_uint32
         _lib = L
     return _lib
 
@@ -217,3 +243,81 @@ def wavefront_frame(trace_input, bases: np.ndarray, blocks: np.ndarray, usages: 
 def _request_dtype():
     return np.dtype([("address", "<u4"), ("tick", "<u2"), ("depth", "u1"), ("pad0", "u1"), ("location", "<u8", (3,)),
                      ("tree_size", "u1"), ("pad1", "u1", (7,))])
+
+
+# ---------------------------------------------------------------------------
+# tree management (prune / graft / collect)
+# ---------------------------------------------------------------------------
+
+UPDATE_INPUT_DTYPE = np.dtype([("base_depth", "u1"), ("pad0", "u1"), ("tick", "<u2"), ("max_child_request_id", "<u4"),
+                               ("memory_size", "<u4"), ("offset", "<u4"), ("graft_size", "<u4")])
+assert UPDATE_INPUT_DTYPE.itemsize == 20
+
+
+def update_input(base_depth: int, tick: int, offset: int = 0, memory_size: int = 0, max_child_request_id: int = 0,
+                 graft_size: int = 0) -> np.ndarray:
+    u = np.zeros(1, UPDATE_INPUT_DTYPE)
+    u["base_depth"], u["tick"], u["offset"] = base_depth, tick, offset
+    u["memory_size"], u["max_child_request_id"], u["graft_size"] = memory_size, max_child_request_id, graft_size
+    return u
+
+
+class TreePool:
+    """Mutable copies of the arrays a management dispatch works on (all numpy, in place)."""
+
+    def __init__(self, bases, blocks, usages, parents=None, residency=None, request_capacity: int = 0,
+                 request_count: int = 0):
+        self.bases = np.ascontiguousarray(bases).copy()
+        self.blocks = np.ascontiguousarray(blocks).copy()
+        self.usages = np.ascontiguousarray(usages).copy()
+        self.parents = None if parents is None else np.ascontiguousarray(parents).copy()
+        n_par = 0 if self.parents is None else self.parents.size
+        self.residency = np.zeros(n_par, np.uint8) if residency is None else np.ascontiguousarray(residency).copy()
+        self.requests = np.zeros(max(request_capacity, 1), dtype=_request_dtype())
+        self.request_capacity = request_capacity
+        self.count = C.c_uint32(request_count)
+
+    def _pool(self) -> Pool:
+        p = Pool()
+        p.bases, p.n_bases = self.bases.ctypes.data, self.bases.size
+        p.blocks, p.n_blocks = self.blocks.ctypes.data, self.blocks.size
+        p.usages, p.n_usages = self.usages.ctypes.data, self.usages.size
+        if self.parents is not None and self.parents.size:
+            p.parents, p.n_parents = self.parents.ctypes.data, self.parents.size
+            p.residency = self.residency.ctypes.data
+        p.requests, p.request_capacity = self.requests.ctypes.data, self.request_capacity
+        p.request_count = C.pointer(self.count)
+        return p
+
+    @property
+    def request_count(self) -> int:
+        return self.count.value
+
+    def prune(self, inp: np.ndarray, records: np.ndarray, surface: np.ndarray, locations: np.ndarray) -> dict:
+        st = TreeStats()
+        loc = np.ascontiguousarray(locations, np.uint64).reshape(-1)
+        rc = lib().orc_prune(C.byref(self._pool()), inp.ctypes.data, records.ctypes.data, records.size,
+                             _ptr(surface) if surface.size else None, surface.size, loc.ctypes.data if loc.size else None,
+                             loc.size // 3, C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"orc_prune failed: {rc}")
+        return {k: getattr(st, k) for k, _ in TreeStats._fields_}
+
+    def collect(self, inp: np.ndarray, need: int) -> np.ndarray:
+        out = np.zeros(max(need, 1), np.uint32)
+        n = lib().orc_collect(C.byref(self._pool()), inp.ctypes.data, need, out.ctypes.data)
+        return out[:n]
+
+    def graft(self, inp: np.ndarray, records: np.ndarray, graft_blocks: np.ndarray, locations: np.ndarray) -> dict:
+        st = TreeStats()
+        loc = np.ascontiguousarray(locations, np.uint64).reshape(-1)
+        rc = lib().orc_graft(C.byref(self._pool()), inp.ctypes.data, records.ctypes.data, records.size,
+                             _ptr(graft_blocks) if graft_blocks.size else None, graft_blocks.size,
+                             loc.ctypes.data if loc.size else None, loc.size // 3, C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"orc_graft failed: {rc}")
+        return {k: getattr(st, k) for k, _ in TreeStats._fields_}
+
+
+def tick_age(tick: int, now: int) -> int:
+    return int(lib().orc_tick_age(tick, now))
