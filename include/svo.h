@@ -307,6 +307,68 @@ typedef struct {
 } svo_frame_stats;
 int svo_frame_stats_get(svo_context *ctx, svo_frame_stats *stats);
 
+/* ---- tree management: prune / graft / collect on the device ----
+ *
+ * The reference kernels (prune, graft, dereference, findAddress: wavefront.cl:1097-1350)
+ * run as one deterministic batch per call:
+ *   1. every record resolves its target against the tree as it stood when the call began
+ *      (Depth 0xFF: the address is a block; otherwise it indexes `locations`, three u64
+ *      per entry, and the target is found by walking from the dense level to Depth; a
+ *      childless block on the way appends a ChildRequest, as findAddress does, and the
+ *      record is skipped; a direct target whose group is free (tick 0) is skipped);
+ *   2. the ticks of every group on a resolved path are set to input->tick (inviolate and
+ *      multi-parent groups keep theirs);
+ *   3. detached subtrees are freed: the child group and every descendant that is neither
+ *      inviolate (65535) nor multi-parent (65534) gets tick 0; a multi-parent group is
+ *      removed from that parent's list instead (ascending (group, parent block) order),
+ *      and becomes single-parent when one parent remains;
+ *   4. fields are written; when several records write the same field of the same target,
+ *      the record with the highest index wins.
+ * Operates on the named buffers Bases, Blocks, Usages (required), Parents and
+ * ParentResidency (optional), ChildRequests + ChildRequestID (optional; capacity
+ * min(MaxChildRequestId, buffer size); requests beyond it are counted, not stored).
+ * Host arrays are copied during the call. */
+typedef struct {
+  uint32_t applied;   /* records applied (grafts: linked) */
+  uint32_t skipped;   /* target free, missing (child requested) or out-run by a later record */
+  uint32_t invalid;   /* an index out of range, BaseDepth outside 1..8, or a malformed graft */
+  uint32_t requests;  /* child requests made, also those beyond capacity */
+  uint32_t collected; /* groups collected (graft, collect) */
+  uint32_t placed;    /* grafts linked into the tree */
+  uint32_t unlinks;   /* multi-parent unlink events */
+  uint32_t reserved;
+} svo_tree_stats;
+
+/* prune, wavefront.cl:1215-1266.  Per record (Pruning.Properties): bit4 BaseBlock writes
+ * Bases[Address] = Chunk and nothing else; bit0 CullChild frees the target's subtree and
+ * sets Child = ChildAddress (and that group's Usage.Parent to the target); bits1-2 == 3 make
+ * the target's group inviolate, == 1 set its tick; bit3 UpdateChunk copies every field of
+ * surface[ColourAddress] except Child and sets Chunk. */
+int svo_prune(svo_context *ctx, const svo_update_input_data *input, const svo_pruning *records, uint32_t n,
+              const svo_block *surface, uint32_t n_surface, const uint64_t *locations, uint32_t n_locations,
+              svo_tree_stats *stats);
+
+/* graft, wavefront.cl:1268-1350.  Each record links a subtree packed in graft_blocks
+ * [GraftDataAddress, +GraftTotalSize) (whole groups; Child fields index graft_blocks, as
+ * WorldManager.UpdateGrafting packs them, WorldManager.cs:84-103) below its target: groups
+ * are collected first (svo_collect, sum of all grafts' groups), grafts take consecutive
+ * runs of the collected list in record order, each graft's groups in pre-order DFS order;
+ * the target's previous subtree is freed.  Collected groups that no linked graft uses go
+ * back to the pool. */
+int svo_graft(svo_context *ctx, const svo_update_input_data *input, const svo_grafting *records, uint32_t n,
+              const svo_block *graft_blocks, uint32_t n_graft_blocks, const uint64_t *locations,
+              uint32_t n_locations, svo_tree_stats *stats);
+
+/* The collection step of graft (wavefront.cl:1277-1305): claims up to `need` groups that are
+ * free (tick 0) or stale (1..65533 and older than input->offset frames on the 1..65533 tick
+ * cycle), in ascending group order, pass by pass with the offset halved after each pass
+ * until it reaches 0.  A stale live group is cut from its parent block and its descendants
+ * are freed.  Claimed groups get tick input->tick and parent UINT32_MAX.  input->memory_size
+ * (blocks; 0 = all of Usages) bounds the groups scanned.  `groups` (host, may be NULL)
+ * receives the claimed group numbers in claim order. */
+int svo_collect(svo_context *ctx, const svo_update_input_data *input, uint32_t need, uint32_t *groups,
+                uint32_t *count);
+
 /* ---- timing on the context's stream (hipEvent based) ---- */
 int svo_timer_begin(svo_context *ctx);
 int svo_timer_end(svo_context *ctx, float *elapsed_ms);

@@ -407,8 +407,18 @@ int orc_graft(orc_pool *p, const orc_update_input *in, const orc_grafting *rec, 
   orc_tree_stats st;
   memset(&st, 0, sizeof st);
   const uint16_t now = in->tick;
-  uint32_t *number = (uint32_t *)malloc(sizeof(uint32_t) * (n_gb / 8u + 1u));
-  uint32_t *parent = (uint32_t *)malloc(sizeof(uint32_t) * (n_gb / 8u + 1u));
+  /* per-record storage of the DFS numbering: records may share graft data */
+  uint32_t *size = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+  uint32_t *where = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+  uint64_t total_size = 0;
+  for (uint32_t r = 0; r < n; r++) {
+    const uint32_t gda = rec[r].graft_data_address, tot = rec[r].graft_total_size;
+    size[r] = (tot >= 8u && (tot & 7u) == 0 && (gda & 7u) == 0 && (uint64_t)gda + tot <= n_gb) ? tot / 8u : 0;
+    where[r] = (uint32_t)total_size;
+    total_size += size[r];
+  }
+  uint32_t *number = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(total_size + 1u));
+  uint32_t *parent = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(total_size + 1u));
   uint32_t *need = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
   uint32_t *start = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
   res_t *res = (res_t *)calloc(n ? n : 1, sizeof(res_t));
@@ -418,9 +428,8 @@ int orc_graft(orc_pool *p, const orc_update_input *in, const orc_grafting *rec, 
   uint32_t total_need = 0;
   for (uint32_t r = 0; r < n; r++) {
     const uint32_t gda = rec[r].graft_data_address, tot = rec[r].graft_total_size;
-    int ok = tot >= 8u && (tot & 7u) == 0 && (gda & 7u) == 0 && (uint64_t)gda + tot <= n_gb;
-    if (ok) ok = graft_structure(gb, gda, tot, number + gda / 8u, parent + gda / 8u);
-    need[r] = ok ? tot / 8u : 0;
+    const int ok = size[r] && graft_structure(gb, gda, tot, number + where[r], parent + where[r]);
+    need[r] = ok ? size[r] : 0;
     start[r] = total_need;
     total_need += need[r];
   }
@@ -478,11 +487,11 @@ int orc_graft(orc_pool *p, const orc_update_input *in, const orc_grafting *rec, 
   st.unlinks = free_subtrees(p, &roots, claimed, now);
   free(roots.v);
 
-  /* copy the graft blocks into their groups and link the top groups */
+  /* copy the graft blocks into their groups */
   for (uint32_t r = 0; r < n; r++) {
     if (!placed[r]) continue;
     const uint32_t gda = rec[r].graft_data_address, ng = need[r];
-    const uint32_t *num = number + gda / 8u, *par = parent + gda / 8u;
+    const uint32_t *num = number + where[r], *par = parent + where[r];
     for (uint32_t q = 0; q < ng; q++) {
       const uint32_t P = L[start[r] + num[q]];
       for (uint32_t k = 0; k < 8; k++) {
@@ -496,8 +505,10 @@ int orc_graft(orc_pool *p, const orc_update_input *in, const orc_grafting *rec, 
       else
         p->usages[P].parent = 8u * L[start[r] + num[par[q] / 8u]] + (par[q] & 7u);
     }
-    p->blocks[res[r].addr].child = 8u * L[start[r]];
   }
+  /* links after every copy (a target may sit in a group another graft filled) */
+  for (uint32_t r = 0; r < n; r++)
+    if (placed[r]) p->blocks[res[r].addr].child = 8u * L[start[r]];
 
   /* groups collected but not used go back to the free pool */
   uint8_t *used = (uint8_t *)calloc(got ? got : 1, 1);
@@ -515,6 +526,8 @@ int orc_graft(orc_pool *p, const orc_update_input *in, const orc_grafting *rec, 
   free(claimed);
   free(number);
   free(parent);
+  free(size);
+  free(where);
   free(need);
   free(start);
   free(res);

@@ -208,6 +208,16 @@ class TraceInput(C.Structure):
     ]
 
 
+class TreeStats(C.Structure):
+    """svo_tree_stats: what one prune / graft / collect call did."""
+
+    _fields_ = [(k, C.c_uint32) for k in ("applied", "skipped", "invalid", "requests", "collected", "placed",
+                                           "unlinks", "reserved")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class UpdateInputData(C.Structure):
     """UpdateInputData, wavefront.cl:77-84 (natural 20-byte layout)."""
 
@@ -325,6 +335,11 @@ _PROTOS = {
     "svo_tree_free": [C.c_void_p],
     "svo_context_load_tree": [C.c_void_p, C.c_void_p, C.c_int, C.c_uint16],
     "svo_camera_lut": [C.POINTER(TraceInput), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "svo_prune": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                  C.c_void_p],
+    "svo_graft": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                  C.c_void_p],
+    "svo_collect": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)],
 }
 
 
@@ -358,6 +373,10 @@ def device_count() -> int:
     n = C.c_int(0)
     _check(lib().svo_device_count(C.byref(n)), "DeviceCount")
     return n.value
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
 
 
 def _as_bytes(data) -> np.ndarray:
@@ -484,6 +503,56 @@ class ComputeManager:
         _check(lib().svo_finish(self._h), "WaitForEvents")
 
     # --- frame-level entry points (extensions) ---
+    @staticmethod
+    def _update_input(update_input) -> UpdateInputData:
+        if isinstance(update_input, UpdateInputData):
+            return update_input
+        raw = np.ascontiguousarray(update_input).view(np.uint8)
+        if raw.size != C.sizeof(UpdateInputData):
+            raise ValueError("UpdateInputData must be 20 bytes")
+        return UpdateInputData.from_buffer_copy(raw.tobytes())
+
+    @staticmethod
+    def _arr(a, dtype):
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(a, dtype=dtype)
+        return a, a.size
+
+    def Prune(self, update_input, records, surface=None, locations=None) -> dict:
+        """svo_prune: Pruning records (PRUNING_DTYPE) with their surface blocks and target
+        locations (three u64 per entry) applied to Bases/Blocks/Usages on the device."""
+        ui = self._update_input(update_input)
+        rec, n = self._arr(records, PRUNING_DTYPE)
+        surf, ns = self._arr(surface, BLOCK_DTYPE)
+        loc = None if locations is None else np.ascontiguousarray(locations, dtype=np.uint64).reshape(-1)
+        st = TreeStats()
+        _check(lib().svo_prune(self._h, C.byref(ui), _ptr(rec) if n else None, n, _ptr(surf) if ns else None, ns,
+                               _ptr(loc) if loc is not None and loc.size else None,
+                               0 if loc is None else loc.size // 3, C.byref(st)), "Prune")
+        return st.as_dict()
+
+    def Graft(self, update_input, records, graft_blocks=None, locations=None) -> dict:
+        """svo_graft: Grafting records (GRAFTING_DTYPE) whose packed subtrees are linked
+        below their targets after collecting the groups they need."""
+        ui = self._update_input(update_input)
+        rec, n = self._arr(records, GRAFTING_DTYPE)
+        gb, ngb = self._arr(graft_blocks, BLOCK_DTYPE)
+        loc = None if locations is None else np.ascontiguousarray(locations, dtype=np.uint64).reshape(-1)
+        st = TreeStats()
+        _check(lib().svo_graft(self._h, C.byref(ui), _ptr(rec) if n else None, n, _ptr(gb) if ngb else None, ngb,
+                               _ptr(loc) if loc is not None and loc.size else None,
+                               0 if loc is None else loc.size // 3, C.byref(st)), "Graft")
+        return st.as_dict()
+
+    def Collect(self, update_input, need: int) -> np.ndarray:
+        """svo_collect: claims up to `need` free or stale groups; returns them in claim order."""
+        ui = self._update_input(update_input)
+        out = np.zeros(max(need, 1), np.uint32)
+        count = C.c_uint32(0)
+        _check(lib().svo_collect(self._h, C.byref(ui), need, out.ctypes.data, C.byref(count)), "Collect")
+        return out[: count.value]
+
     def LoadTree(self, tree: "Octree", with_usages: bool = True, usage_tick: int = 0) -> None:
         _check(lib().svo_context_load_tree(self._h, tree.handle, 1 if with_usages else 0, usage_tick), "LoadTree")
 

@@ -170,6 +170,8 @@ int svo_destroy(svo_context *ctx) {
   }
   (void)hipEventDestroy(ctx->timer_begin);
   (void)hipEventDestroy(ctx->timer_end);
+  for (auto &b : ctx->tree_scratch) b.reset();
+  if (ctx->tree_host) (void)hipHostFree(ctx->tree_host);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SVO_OK;

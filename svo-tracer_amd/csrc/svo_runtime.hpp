@@ -52,6 +52,8 @@ struct FrameBuffers {
 
 struct Context;
 int update_camera_table(Context *ctx, const svo_trace_input &in);
+// KernelName.Prune / KernelName.Graft behind svo_enqueue (arguments bound with SetArg).
+int enqueue_tree(Context *ctx, int kernel, uint32_t n);
 int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb);
 
 struct Context {
@@ -67,6 +69,9 @@ struct Context {
   std::shared_ptr<DeviceBuffer> scratch_status, scratch_requests, scratch_small, scratch_counts, scratch_lut;
   // scratch of the kernel-by-kernel path
   std::shared_ptr<DeviceBuffer> wave_codes, wave_values, wave_staging, wave_counts;
+  // scratch of tree management (prune / graft / collect) and its pinned counter mirror
+  std::shared_ptr<DeviceBuffer> tree_scratch[24];
+  uint32_t *tree_host = nullptr;
   // camera table cache key
   uint32_t lut_w = 0, lut_h = 0;
   float lut_fov0 = 0.f, lut_fov1 = 0.f;

@@ -1,0 +1,119 @@
+// Launch interface of the tree-management kernels (prune, graft, collect).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "svo.h"
+
+namespace svo {
+
+// The mutable pool one management dispatch works on (device pointers).
+struct PoolView {
+  uint16_t *bases;
+  uint32_t n_bases;
+  svo_block *blocks;
+  uint32_t n_blocks;
+  svo_usage *usages;
+  uint32_t n_usages;
+  svo_parent *parents;  // null when no parent list is bound
+  uint32_t n_parents;
+  uint8_t *residency;   // one byte per parent element, may be null
+  svo_child_request *requests;
+  uint32_t request_capacity;
+  uint32_t *request_count;  // null: requests are only counted
+  uint32_t B;
+  uint32_t now;
+};
+
+// Per-record resolution of a target (findAddress, wavefront.cl:1191-1211).
+struct ResolveOut {
+  uint32_t status;
+  uint32_t addr;
+  uint32_t level;  // kResMiss: level of the childless block
+  uint32_t placed; // graft: linked into the tree
+};
+enum : uint32_t { kResOk = 0, kResMiss = 1, kResSkip = 2, kResInvalid = 3, kResBase = 4 };
+
+// Slots of the u32 counter block of a management dispatch.
+enum : uint32_t {
+  kTcFront0 = 0,      // frontier sizes (ping-pong)
+  kTcFront1 = 1,
+  kTcEvents = 2,      // multi-parent unlink events
+  kTcOverflow = 3,    // frontier overflow flag
+  kTcTotal = 4,       // scan totals read by the host
+  kTcApplied = 8,     // statistics (svo_tree_stats order)
+  kTcSkipped = 9,
+  kTcInvalid = 10,
+  kTcRequests = 11,
+  kTcCollected = 12,
+  kTcPlaced = 13,
+  kTcUnlinks = 14,
+  kTcSlots = 64
+};
+
+// Scratch of a management dispatch (device pointers; sizes in elements).
+struct TreeScratch {
+  uint32_t *counters;            // kTcSlots
+  ResolveOut *res;               // per record
+  unsigned long long *keys;      // winner table
+  uint32_t *vals;
+  uint32_t table_mask;
+  uint2 *front[2];
+  uint32_t front_capacity;
+  uint2 *events;
+  uint32_t event_capacity;       // power of two, at most kUnlinkSortMax
+  uint32_t *claimed;             // bitmap over groups
+  uint32_t claimed_words;
+  uint32_t *chunk_counts;        // collection scan, one per chunk of groups
+  uint32_t *list;                // collected groups
+  uint32_t list_capacity;
+  uint32_t *size, *where, *need, *start;  // graft: per record
+  uint32_t *number, *parent;     // graft: per graft group (DFS numbering)
+};
+
+constexpr uint32_t kUnlinkSortMax = 4096;
+constexpr uint32_t kCollectChunk = 1024;  // groups per collection work-group
+constexpr uint32_t kMaxFreeLevels = 64;
+
+int tree_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+int tree_fill_u64(unsigned long long *p, unsigned long long value, size_t n, hipStream_t s);
+
+int launch_prune_resolve(const PoolView &p, const svo_pruning *rec, uint32_t n, uint32_t n_surface,
+                         const uint64_t *locs, uint32_t n_locs, const TreeScratch &t, hipStream_t s);
+int launch_graft_resolve(const PoolView &p, const svo_grafting *rec, uint32_t n, const uint64_t *locs,
+                         uint32_t n_locs, const TreeScratch &t, hipStream_t s);
+// child requests of the missed records, in record order, then the ticks along every walk
+int launch_requests_and_touches(const PoolView &p, const uint8_t *depths, uint32_t depth_stride,
+                                const uint32_t *addresses, uint32_t address_stride, uint32_t n,
+                                const uint64_t *locs, const TreeScratch &t, hipStream_t s);
+int launch_prune_winners(const PoolView &p, const svo_pruning *rec, uint32_t n, const TreeScratch &t,
+                         hipStream_t s);
+int launch_prune_roots(const PoolView &p, const svo_pruning *rec, uint32_t n, const TreeScratch &t,
+                       hipStream_t s);
+// one level of the free walk: reads front[in], writes front[1-in]
+int launch_free_level(const PoolView &p, const TreeScratch &t, uint32_t in, bool use_stop, hipStream_t s);
+int launch_unlink(const PoolView &p, const TreeScratch &t, hipStream_t s);
+int launch_prune_apply(const PoolView &p, const svo_pruning *rec, uint32_t n, const svo_block *surface,
+                       const TreeScratch &t, hipStream_t s);
+
+// collection: one pass with `offset`; candidates ranked in ascending group order
+int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, const TreeScratch &t,
+                         hipStream_t s);
+int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, uint32_t from, uint32_t limit,
+                           const TreeScratch &t, hipStream_t s);
+int launch_collect_claim(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s);
+int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s);
+
+// graft
+int launch_graft_sizes(const svo_grafting *rec, uint32_t n, uint32_t n_gb, const TreeScratch &t, hipStream_t s);
+int launch_graft_structure(const svo_grafting *rec, uint32_t n, const svo_block *gb, uint32_t n_gb,
+                           const TreeScratch &t, hipStream_t s);
+int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, hipStream_t s);
+int launch_graft_winners(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
+int launch_graft_roots(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
+int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, const svo_block *gb,
+                       const TreeScratch &t, hipStream_t s);
+int launch_graft_release(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
+
+}  // namespace svo

@@ -142,11 +142,30 @@ trace_frame_kernel(const FrameParams p) {
     active = false;
   };
 
+  // A cone that ends only records how (`fin`: bit0 pending, bit1 material, bit2 child
+  // request, bit3 TreeSize, bit4 step cap; the block address in node_addr, the block in
+  // node); its outputs are written when the wave next refills or exits, so the shading
+  // and output code runs once for all lanes that ended in between instead of once per
+  // step in which any lane ended.
+  uint32_t fin = 0;
+  auto retire = [&](bool material, uint32_t addr, bool child, bool tree_size, bool capped) {
+    node_addr = addr;
+    fin = 1u | (material ? 2u : 0u) | (child ? 4u : 0u) | (tree_size ? 8u : 0u) | (capped ? 16u : 0u);
+    active = false;
+  };
+  auto flush = [&]() {
+    if (fin) {
+      finish((fin & 2u) != 0u, node_addr, node, (fin & 4u) != 0u, (fin & 8u) != 0u, (fin & 16u) != 0u);
+      fin = 0;
+    }
+  };
+
   for (;;) {
     // ---- refill idle lanes with new cones (wave-uniform control flow) ----
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (fetching && (n_idle >= p.refill || n_idle == 64u)) {
+      flush();
       // claim a slice of `chunk` work items with one atomic when the wave's slice is used up
       while (cur >= cur_end && range_step < 8u) {
         const uint32_t r = (xcd + range_step) & 7u;
@@ -196,7 +215,7 @@ trace_frame_kernel(const FrameParams p) {
             } else {
               depth = 0;
               lx = ly = lz = 0;
-              finish(false, kU32Max, node, false, false, false);
+              retire(false, kU32Max, false, false, false);
             }
           }
         }
@@ -211,7 +230,7 @@ trace_frame_kernel(const FrameParams p) {
     // ---- one step of the reference state machine ----
     if (nvis >= kStepCap) {
       depth = 0;
-      finish(false, kU32Max, node, false, false, true);
+      retire(false, kU32Max, false, false, true);
       continue;
     }
     const uint32_t d = depth;
@@ -246,7 +265,7 @@ trace_frame_kernel(const FrameParams p) {
       traverse_chunk(d, lx, ly, lz, dx, dy, dz, ix, iy, iz, signs);
       if (leaving(lx, ly, lz, signs)) {
         depth = 0;
-        finish(false, kU32Max, node, false, false, false);
+        retire(false, kU32Max, false, false, false);
         continue;
       }
       uint32_t nd = compare_positions(d, px, py, pz, lx, ly, lz, signs);
@@ -274,14 +293,15 @@ trace_frame_kernel(const FrameParams p) {
       if (cone > exp2_neg(B + 2u)) {
         depth = B + 2u;
         const uint32_t a = stack_get(B + 2u);
-        finish(true, a, load_block(p.blocks, p.n_blocks, a), false, false, false);
+        node = load_block(p.blocks, p.n_blocks, a);
+        retire(true, a, false, false, false);
       } else if (cone > exp2_neg(d)) {
         depth = d - 1u;
         node_addr = stack_get(d - 1u);
         node = load_block(p.blocks, p.n_blocks, node_addr);
       } else if (block_child(node) == kU32Max) {
         // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
-        finish(true, node_addr, node, true, cone <= exp2_neg(d + 1u), false);
+        retire(true, node_addr, true, cone <= exp2_neg(d + 1u), false);
       } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
         const uint32_t a = block_child(node) + oct;
         depth = d + 1u;
@@ -289,10 +309,12 @@ trace_frame_kernel(const FrameParams p) {
         node_addr = a;
         node = load_block(p.blocks, p.n_blocks, a);
       } else {
-        finish(true, node_addr, node, false, false, false);
+        retire(true, node_addr, false, false, false);
       }
     }
   }
+
+  flush();
 
   // frame totals: one 64-bit atomic per wave
   uint32_t vb = tot_vis - tot_blk, vk = tot_blk;

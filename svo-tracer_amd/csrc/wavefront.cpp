@@ -337,6 +337,9 @@ int enqueue_wavefront(Context *ctx, int kernel, uint32_t dims, const size_t *g) 
       return enqueue_draw(ctx, r, gsx, gsy);
     case SVO_KERNEL_TRACE:
       return enqueue_trace(ctx, r, gsx, gsy);
+    case SVO_KERNEL_PRUNE:
+    case SVO_KERNEL_GRAFT:
+      return enqueue_tree(ctx, kernel, gsx * gsy);
     case SVO_KERNEL_TEST:
       // the reference's argument table names a kernel "test" that its programs do not define
       return fail(SVO_ERR_UNKNOWN_KERNEL, "CreateKernel:Test: no kernel named test");

@@ -66,6 +66,9 @@ def test_errors_raise_with_method_and_code(svo):
     assert "null context" in svo.last_error()
     assert lib.svo_buffer_init(None, 0, 16, None, 0) == -1
     assert lib.svo_trace_frame(None, None, None) == -1
+    assert lib.svo_prune(None, None, None, 0, None, 0, None, 0, None) == -1
+    assert lib.svo_graft(None, None, None, 0, None, 0, None, 0, None) == -1
+    assert lib.svo_collect(None, None, 0, None, None) == -1
 
 
 def test_create_without_device_fails_cleanly(svo):
