@@ -72,6 +72,149 @@ def cpu_baseline(svo, tree, cams, seconds: float):
     }
 
 
+def graph_opts(opts: dict) -> dict:
+    return {k: v for k, v in opts.items() if v}
+
+
+def batched_leg(svo, cm, steps: int, warmup: int, usages0, opts) -> dict:
+    """The 64-camera set of uneven resolution (log-uniform 256^2..1920x1080, apertures 1-4 px,
+    seed 11) over the same scene: one svo_trace_batch launch per step, beside the same
+    cameras traced with one launch each, both submitted directly and as replayed HIP graphs."""
+    B = svo.BufferName
+    cams, apertures = svo.scenes.config4_cameras(count=64, tick=1)
+    n = sum(c.screen_size[0] * c.screen_size[1] for c in cams)
+    cm.InitDeviceBuffer(B.OutputImage, n, 16)
+    cm.InitDeviceBuffer(B.HitIndices, n, 4)
+    cm.InitDeviceBuffer(B.VisitCounts, n, 4)
+
+    def frame_cams(f):
+        for c in cams:
+            c.tick = f + 1
+        return cams
+
+    def timed(fn):
+        cm.WriteBuffer(B.Usages, usages0)
+        for f in range(warmup):
+            fn(f)
+        cm.Wait()
+        cm.WriteBuffer(B.Usages, usages0)
+        cm.Wait()
+        cm.TimerBegin()
+        for f in range(steps):
+            fn(f)
+        return cm.TimerEnd() / steps
+
+    def batch(f):
+        cm.TraceBatch(frame_cams(f), apertures, **opts)
+
+    def loop(f):
+        for c, a in zip(frame_cams(f), apertures):
+            cm.TraceBatch([c], [a], **opts)
+
+    ms_batch = timed(batch)
+    ms_loop = timed(loop)
+    # the same two submissions captured once into HIP graphs and replayed per step
+    g_batch = cm.CaptureFrame(frame_cams(0), apertures, per_camera=False, **graph_opts(opts))
+    g_loop = cm.CaptureFrame(frame_cams(0), apertures, per_camera=True, **graph_opts(opts))
+    try:
+        ms_batch_graph = timed(lambda f: g_batch.Launch(frame_cams(f)))
+        ms_loop_graph = timed(lambda f: g_loop.Launch(frame_cams(f)))
+    finally:
+        g_batch.Dispose()
+        g_loop.Dispose()
+    rate = lambda ms: round(n / (ms / 1e3) / 1e6, 2)
+    return {
+        "value": rate(ms_batch_graph),
+        "unit": "Mcones/s",
+        "ms_per_step": round(ms_batch_graph, 3),
+        "cameras": len(cams),
+        "cones_per_step": n,
+        "submission": "hipGraph replay of one batched launch",
+        "direct_launch": {"value": rate(ms_batch), "ms_per_step": round(ms_batch, 3)},
+        "per_camera_loop": {"value": rate(ms_loop), "ms_per_step": round(ms_loop, 3)},
+        "per_camera_loop_graph": {"value": rate(ms_loop_graph), "ms_per_step": round(ms_loop_graph, 3)},
+        "workload": "64 cameras, resolutions log-uniform 256x256..1920x1080, apertures 1-4 px, seed 11, "
+                    "one launch per step vs one launch per camera",
+    }
+
+
+def adjoint_leg(svo, steps: int, fit_steps: int) -> dict:
+    """Forward + adjoint on the depth-10 sphere+noise scene (seed 3), 32 views of 512x512:
+    targets rendered from the true leaf colours and opacities, parameters perturbed by
+    N(0, 0.1) (colour) and N(0, 0.05) (opacity), clipped to [0, 1]; L2 loss.  Times the
+    plain forward of the same views, the compositing forward and forward + adjoint, then
+    runs the Adam fit."""
+    B = svo.BufferName
+    spec = svo.scenes.config5_scene()
+    tree = spec.build()
+    views = svo.scenes.config5_cameras()
+    for v in views:
+        v.tick = 1
+    n = sum(v.screen_size[0] * v.screen_size[1] for v in views)
+    blocks = tree.blocks
+    truth = np.zeros((blocks.size, 4), np.float32)
+    truth[:, 0] = blocks["colour_r"] / np.float32(255.0)
+    truth[:, 1] = blocks["colour_g"] / np.float32(255.0)
+    truth[:, 2] = blocks["colour_b"] / np.float32(255.0)
+    truth[:, 3] = blocks["opacity"] / np.float32(255.0)
+    rng = np.random.default_rng(3)
+    params = truth.copy()
+    params[:, :3] = np.clip(params[:, :3] + rng.normal(0, 0.1, (blocks.size, 3)), 0, 1)
+    params[:, 3] = np.clip(params[:, 3] + rng.normal(0, 0.05, blocks.size), 0, 1)
+    cm = svo.ComputeManager(0)
+    try:
+        cm.LoadTree(tree, with_usages=False)
+        cm.InitDeviceBuffer(B.OutputImage, n, 16)
+        cm.InitBuffer(B.LeafParams, truth)
+        cm.Composite(views)
+        target = cm.ReadBuffer(B.OutputImage, np.float32, 4 * n)
+        cm.InitBuffer(B.TargetImage, target)
+        cm.InitBuffer(B.LeafParams, params)
+        cm.InitBuffer(B.LeafGradients, np.zeros_like(params))
+        cm.InitBuffer(B.AdamM, np.zeros_like(params))
+        cm.InitBuffer(B.AdamV, np.zeros_like(params))
+
+        def timed(fn):
+            for _ in range(2):
+                fn()
+            cm.Wait()
+            cm.TimerBegin()
+            for _ in range(steps):
+                fn()
+            return cm.TimerEnd() / steps
+
+        ms_plain = timed(lambda: cm.TraceBatch(views))
+        ms_fwd = timed(lambda: cm.Composite(views))
+        ms_adj = timed(lambda: cm.Composite(views, adjoint=True))
+        cm.InitBuffer(B.LeafParams, params)
+        cm.InitBuffer(B.LeafGradients, np.zeros_like(params))
+        losses = []
+        t0 = time.perf_counter()
+        for step in range(1, fit_steps + 1):
+            losses.append(cm.Composite(views, adjoint=True))
+            cm.AdamStep(step, learning_rate=0.01)
+        cm.Wait()
+        fit_s = time.perf_counter() - t0
+        final = cm.Composite(views)
+    finally:
+        cm.Dispose()
+    rate = lambda ms: round(n / (ms / 1e3) / 1e6, 2)
+    return {
+        "value": rate(ms_fwd + ms_adj),
+        "unit": "Mcones/s",
+        "ms_per_step": round(ms_fwd + ms_adj, 3),
+        "workload": "depth-10 sphere+noise (seed 3), 32 views of 512x512, 4 layers, L2 loss",
+        "cones_per_step": n,
+        "plain_forward_ms": round(ms_plain, 3),
+        "composite_forward_ms": round(ms_fwd, 3),
+        "forward_plus_adjoint_ms": round(ms_fwd + ms_adj, 3),
+        "adjoint_pass_ms": round(ms_adj, 3),
+        "cost_vs_plain_forward": round((ms_fwd + ms_adj) / ms_plain, 3),
+        "adam_fit": {"steps": fit_steps, "loss_first": round(losses[0], 4), "loss_last": round(final, 4),
+                     "seconds": round(fit_s, 2)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +229,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves", type=int, default=0, help="register budget of the trace kernel (0 = default, or 6)")
     ap.add_argument("--chunk", type=int, default=0, help="cones a wave claims per atomic")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -200,6 +345,15 @@ def main() -> int:
             "traffic": traffic,
         },
     }
+    if not args.no_batched:
+        log("batched 64-camera leg...")
+        line["batched"] = batched_leg(svo, cm, max(3, min(K, 10)), 2, usages0, opts)
+        log(f"batched: {line['batched']['value']} Mcones/s, per-camera loop "
+            f"{line['batched']['per_camera_loop']['value']} Mcones/s")
+    if not args.no_adjoint:
+        log("forward + adjoint leg...")
+        line["adjoint"] = adjoint_leg(svo, max(3, min(K, 10)), 200)
+        log(f"adjoint: {line['adjoint']['value']} Mcones/s, {line['adjoint']['cost_vs_plain_forward']}x plain forward")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample)...")
         line["cpu_baseline"] = cpu_baseline(svo, tree, cams, args.cpu_seconds)

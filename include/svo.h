@@ -125,6 +125,13 @@ enum {
   SVO_BUF_VISIT_COUNTS = 66, /* u32 per ray: base + block steps taken */
   SVO_BUF_VISIT_HASHES = 67, /* u64 per ray: FNV-1a-64 of the visit words (depth<<32 | index) */
   SVO_BUF_RAY_STATUS = 68,   /* u8 per ray: bit0 material, bit1 child requested, bit2 step cap */
+  /* differentiable compositing (svo_composite, svo_adam_step) */
+  SVO_BUF_LEAF_PARAMS = 69,    /* float4 per block: colour r, g, b and opacity in [0, 1] */
+  SVO_BUF_TARGET_IMAGE = 70,   /* float4 per cone: target colour of the L2 loss */
+  SVO_BUF_LEAF_GRADIENTS = 71, /* float4 per block: accumulated dLoss/dLeafParams */
+  SVO_BUF_LAYERS = 72,         /* max_layers u32 per cone: composited block addresses */
+  SVO_BUF_ADAM_M = 73,         /* float4 per block: Adam first moment */
+  SVO_BUF_ADAM_V = 74,         /* float4 per block: Adam second moment */
   SVO_BUF_NAME_LIMIT = 80
 };
 
@@ -288,7 +295,9 @@ typedef struct {
 enum {
   SVO_FRAME_STATE = 1u << 0,  /* also write final Locations (ulong3, 32-B stride) and Depths */
   SVO_FRAME_QUEUES = 1u << 1, /* also write BackgroundQueue/MaterialQueue (+ their ID counters) */
-  SVO_FRAME_NO_XCD_SPLIT = 1u << 2
+  SVO_FRAME_NO_XCD_SPLIT = 1u << 2,
+  SVO_FRAME_CAMERA_TABLE = 1u << 3 /* read camera parameters from a device table (the batched kernel) also
+                                      for a single camera */
 };
 
 /* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block
@@ -297,6 +306,53 @@ enum {
  * ChildRequestID (if bound); writes the per-ray extension buffers that are bound.
  * Queue contents are in ascending ray-id order. */
 int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options);
+
+/* Several cameras of uneven resolution in one persistent launch (no reference
+ * counterpart; the batched form of svo_trace_frame).  The per-cone buffers hold the
+ * cameras' cones concatenated in camera order (camera c's cone (x, y) at
+ * sum_{j<c} W_j*H_j + x + y*W_c); child requests and queues are in ascending concatenated
+ * index.  apertures[c] (NULL: all 1) is the cone aperture in pixels: cone FoV =
+ * (apertures[c] * FoV.x) / W_c.  All cameras share BaseDepth and Tick; at most 256. */
+int svo_trace_batch(svo_context *ctx, const svo_trace_input *inputs, const float *apertures, uint32_t n_cameras,
+                    const svo_frame_options *options);
+
+/* A frame's submission captured into a HIP graph (no reference counterpart): the camera
+ * upload, counter reset, trace kernel and compaction of svo_trace_batch (per_camera = 0) or
+ * of one svo_trace_batch per camera (per_camera = 1, each camera's cones at index 0), with
+ * scratch owned by the graph.  Replays take new poses and a new tick; resolutions, FoVs,
+ * BaseDepth, MaxChildRequestId and the bound buffers must stay as captured. */
+typedef struct svo_frame_graph svo_frame_graph;
+int svo_frame_graph_create(svo_context *ctx, const svo_trace_input *inputs, const float *apertures,
+                           uint32_t n_cameras, const svo_frame_options *options, uint32_t per_camera,
+                           svo_frame_graph **out);
+int svo_frame_graph_launch(svo_context *ctx, svo_frame_graph *graph, const svo_trace_input *inputs);
+int svo_frame_graph_destroy(svo_context *ctx, svo_frame_graph *graph);
+
+/* ---- differentiable compositing (no reference counterpart) ----
+ * Every material hit of a cone (the block the reference would shade) becomes a layer with
+ * the block's LeafParams (r, g, b, a); the cone continues past the hit cell, as it would
+ * past an empty octant, until it has max_layers layers or leaves the volume.  Colour =
+ * sum_i T_i a_i c_i + T_end * sky, T_1 = 1, T_{i+1} = T_i (1 - a_i), in FP32; with every
+ * opacity 1 it is the reference's first-hit colour.  OutputImage receives the colour; with
+ * a TargetImage the L2 loss sum (C - target)^2 is returned; SVO_COMPOSITE_ADJOINT adds
+ * dLoss/dLeafParams to LeafGradients (the traversal is recomputed, no per-sample state is
+ * stored); SVO_COMPOSITE_LAYERS writes each cone's layer addresses to Layers. */
+typedef struct {
+  uint32_t max_layers; /* 1..4, 0 = 4 */
+  uint32_t flags;      /* SVO_COMPOSITE_* */
+  uint32_t reserved[2];
+} svo_composite_options;
+enum { SVO_COMPOSITE_ADJOINT = 1u << 0, SVO_COMPOSITE_LAYERS = 1u << 1 };
+int svo_composite(svo_context *ctx, const svo_trace_input *views, const float *apertures, uint32_t n_views,
+                  const svo_composite_options *options, double *loss);
+
+/* One Adam step over LeafParams from LeafGradients (moments in AdamM / AdamV, zero them
+ * before step 1); parameters are clamped to [0, 1] and LeafGradients cleared. */
+typedef struct {
+  float learning_rate, beta1, beta2, epsilon;
+  uint32_t step; /* 1, 2, ... */
+} svo_adam_options;
+int svo_adam_step(svo_context *ctx, const svo_adam_options *options);
 
 /* Frame totals of the last svo_trace_frame: base steps, block steps, rays, cones. */
 typedef struct {

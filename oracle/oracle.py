@@ -90,6 +90,16 @@ def lib() -> C.CDLL:
                                           C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(C.c_uint32),
                                           C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(FrameOut)]
         L.orc_wavefront_frame.restype = C.c_int
+        L.orc_trace_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.POINTER(FrameOut), C.c_int]
+        L.orc_trace_batch.restype = C.c_int
+        L.orc_composite_layers.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                           C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_composite_layers.restype = C.c_int
+        L.orc_composite_loss.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_composite_loss.restype = C.c_double
         L.orc_prune.argtypes = [C.POINTER(Pool), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                 C.c_uint32, C.POINTER(TreeStats)]
         L.orc_prune.restype = C.c_int
@@ -195,6 +205,30 @@ def trace_frame(trace_input, bases: np.ndarray, blocks: np.ndarray, usages: Opti
                                C.byref(count), C.byref(fo), threads)
     if rc != 0:
         raise RuntimeError(f"orc_trace_frame failed: {rc}")
+    o["requests"] = req[: min(count.value, request_capacity)]
+    o["request_count"] = count.value
+    o["base_steps"] = fo.base_steps
+    o["block_steps"] = fo.block_steps
+    o["groups_touched"] = fo.groups_touched
+    return o
+
+
+def trace_batch(trace_inputs, bases: np.ndarray, blocks: np.ndarray, usages: Optional[np.ndarray] = None,
+                apertures=None, request_capacity: int = 0, threads: int = 1) -> dict:
+    """Several cameras in one frame: per-cone outputs concatenated in camera order."""
+    n_cams = len(trace_inputs)
+    n = sum(t.screen_size[0] * t.screen_size[1] for t in trace_inputs)
+    o, fo = _outputs(n, 0)
+    ins = (type(trace_inputs[0]) * n_cams)(*trace_inputs)
+    ap = None if apertures is None else np.ascontiguousarray(apertures, dtype=np.float32)
+    req = np.zeros(max(request_capacity, 1), dtype=_request_dtype())
+    count = C.c_uint32(0)
+    rc = lib().orc_trace_batch(C.cast(ins, C.c_void_p), None if ap is None else ap.ctypes.data, n_cams,
+                               bases.ctypes.data, bases.size, blocks.ctypes.data, blocks.size, _ptr(usages),
+                               0 if usages is None else usages.size, req.ctypes.data, request_capacity, C.byref(count),
+                               C.byref(fo), threads)
+    if rc != 0:
+        raise RuntimeError(f"orc_trace_batch failed: {rc}")
     o["requests"] = req[: min(count.value, request_capacity)]
     o["request_count"] = count.value
     o["base_steps"] = fo.base_steps
@@ -321,3 +355,43 @@ class TreePool:
 
 def tick_age(tick: int, now: int) -> int:
     return int(lib().orc_tick_age(tick, now))
+
+
+# ---------------------------------------------------------------------------
+# differentiable compositing
+# ---------------------------------------------------------------------------
+
+def composite_layers(trace_inputs, bases: np.ndarray, blocks: np.ndarray, max_layers: int, apertures=None,
+                     threads: int = 1):
+    """Layer lists (block addresses, UINT32_MAX padded) and background colours of every cone of
+    the given views, concatenated view by view."""
+    n_views = len(trace_inputs)
+    n = sum(t.screen_size[0] * t.screen_size[1] for t in trace_inputs)
+    ins = (type(trace_inputs[0]) * n_views)(*trace_inputs)
+    ap = None if apertures is None else np.ascontiguousarray(apertures, dtype=np.float32)
+    layers = np.zeros((n, max_layers), np.uint32)
+    bg = np.zeros((n, 3), np.float32)
+    rc = lib().orc_composite_layers(C.cast(ins, C.c_void_p), None if ap is None else ap.ctypes.data, n_views,
+                                    bases.ctypes.data, bases.size, blocks.ctypes.data, blocks.size, max_layers,
+                                    layers.ctypes.data, bg.ctypes.data, threads)
+    if rc != 0:
+        raise RuntimeError(f"orc_composite_layers failed: {rc}")
+    return layers, bg
+
+
+def composite_loss(layers: np.ndarray, bg: np.ndarray, params: np.ndarray, target: Optional[np.ndarray] = None,
+                   rays: Optional[np.ndarray] = None, want_image: bool = False, want_grads: bool = False):
+    """FP64 compositing of layer lists: (loss, image or None, grads or None)."""
+    layers = np.ascontiguousarray(layers, np.uint32)
+    bg = np.ascontiguousarray(bg, np.float32)
+    params = np.ascontiguousarray(params, np.float64)
+    n_rays = layers.shape[0] if rays is None else len(rays)
+    r = None if rays is None else np.ascontiguousarray(rays, np.uint32)
+    t = None if target is None else np.ascontiguousarray(target, np.float64)
+    image = np.zeros((layers.shape[0], 3), np.float64) if want_image else None
+    grads = np.zeros_like(params) if want_grads else None
+    loss = lib().orc_composite_loss(layers.ctypes.data, bg.ctypes.data, None if r is None else r.ctypes.data, n_rays,
+                                    layers.shape[1], params.ctypes.data, params.shape[0],
+                                    None if t is None else t.ctypes.data, None if image is None else image.ctypes.data,
+                                    None if grads is None else grads.ctypes.data)
+    return float(loss), image, grads

@@ -265,7 +265,8 @@ typedef struct {
 
 /* setupInitialTrace + traceIntoVolume, wavefront.cl:563-595, 1372-1381, with the
  * trig of pixel (x,y) supplied as (cu,su) for its column and (cv,sv) for its row. */
-static int init_ray_trig(const orc_trace_input *in, float cu, float su, float cv, float sv, ray_t *r) {
+static int init_ray_trig(const orc_trace_input *in, float aperture, float cu, float su, float cv, float sv,
+                         ray_t *r) {
   float m0 = cu * cv;
   float m1 = (0.0f - su) * cv;
   float m2 = sv;
@@ -278,7 +279,8 @@ static int init_ray_trig(const orc_trace_input *in, float cu, float su, float cv
     r->sign[a] = r->dir[a] >= 0.0f;
     r->origin[a] = in->origin[a];
   }
-  r->fov = in->fov[0] / (float)in->screen_size[0];
+  /* cone aperture: `aperture` pixels (1 in the reference; (1*FoV)/W == FoV/W exactly) */
+  r->fov = (aperture * in->fov[0]) / (float)in->screen_size[0];
   r->dof[0] = in->dof[0];
   r->dof[1] = in->dof[1];
   r->loc[0] = r->loc[1] = r->loc[2] = 0;
@@ -290,16 +292,16 @@ static int init_ray_trig(const orc_trace_input *in, float cu, float su, float cv
   return 0;
 }
 
-static int init_ray_lut(const orc_trace_input *in, const float *cu, const float *su, const float *cv,
-                        const float *sv, uint32_t x, uint32_t y, ray_t *r) {
-  return init_ray_trig(in, cu[x], su[x], cv[y], sv[y], r);
+static int init_ray_lut(const orc_trace_input *in, float aperture, const float *cu, const float *su,
+                        const float *cv, const float *sv, uint32_t x, uint32_t y, ray_t *r) {
+  return init_ray_trig(in, aperture, cu[x], su[x], cv[y], sv[y], r);
 }
 
 int orc_init_ray(const orc_trace_input *in, uint32_t x, uint32_t y, float dir[3], float inv[3],
                  uint64_t loc[3], float *cone_fov) {
   double u = (double)cam_u(in, x), v = (double)cam_v(in, y);
   ray_t r;
-  int hit = init_ray_trig(in, (float)cos(u), (float)sin(u), (float)cos(v), (float)sin(v), &r);
+  int hit = init_ray_trig(in, 1.0f, (float)cos(u), (float)sin(u), (float)cos(v), (float)sin(v), &r);
   for (int a = 0; a < 3; a++) {
     dir[a] = r.dir[a];
     inv[a] = r.inv[a];
@@ -534,40 +536,29 @@ static void apply_usage(orc_usage *usages, uint32_t n_usages, const uint8_t *tou
   if (count) *count = c;
 }
 
-int orc_trace_frame(const orc_trace_input *in, const uint16_t *bases, uint32_t n_bases,
-                    const orc_block *blocks, uint32_t n_blocks, orc_usage *usages, uint32_t n_usages,
-                    orc_child_request *requests, uint32_t request_capacity, uint32_t *request_count,
-                    orc_frame_out *out, int threads) {
-  int rc = check_input(in, n_bases);
-  if (rc) return rc;
+/* One camera of a frame: traces every cone, writes outputs at `offset + id`, flags child
+ * requests in req_flag/req_tmp at the same index and marks touched usage groups. */
+static int frame_core(const orc_trace_input *in, float aperture, const tree_t *t, size_t offset, uint8_t *req_flag,
+                      orc_child_request *req_tmp, orc_frame_out *out, int threads, uint64_t *base_sum,
+                      uint64_t *block_sum) {
   const uint32_t W = in->screen_size[0], H = in->screen_size[1];
   const size_t N = (size_t)W * H;
   float *lut = (float *)malloc(sizeof(float) * 2 * ((size_t)W + H));
-  uint8_t *touched = usages ? (uint8_t *)calloc(n_usages ? n_usages : 1, 1) : NULL;
-  uint8_t *req_flag = (uint8_t *)calloc(N, 1);
-  orc_child_request *req_tmp = (orc_child_request *)malloc(sizeof(orc_child_request) * (N ? N : 1));
-  if (!lut || (usages && !touched) || !req_flag || !req_tmp) {
-    free(lut);
-    free(touched);
-    free(req_flag);
-    free(req_tmp);
-    return -10;
-  }
+  if (!lut) return -10;
   float *cu = lut, *su = lut + W, *cv = lut + 2 * W, *sv = lut + 2 * W + H;
   orc_camera_lut(in, cu, su, cv, sv);
-  tree_t t;
-  setup_tree(&t, in, bases, n_bases, blocks, n_blocks, usages, n_usages, touched);
   uint64_t base_total = 0, block_total = 0;
 #ifdef _OPENMP
   if (threads < 1) threads = 1;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads) reduction(+ : base_total, block_total)
 #endif
   for (long long i = 0; i < (long long)N; i++) {
-    uint32_t id = (uint32_t)i, x = id % W, y = id / W;
+    uint32_t x = (uint32_t)i % W, y = (uint32_t)i / W;
+    size_t id = offset + (size_t)i;
     ray_t r;
     acc_t acc;
-    acc_init(&acc, out, id);
-    int q = init_ray_lut(in, cu, su, cv, sv, x, y, &r) ? Q_BASE : Q_BACKGROUND;
+    acc_init(&acc, out, (uint32_t)id);
+    int q = init_ray_lut(in, aperture, cu, su, cv, sv, x, y, &r) ? Q_BASE : Q_BACKGROUND;
     int child = 0, capped = 0;
     while (q == Q_BASE || q == Q_BLOCK) {
       if (acc.visits >= STEP_CAP) {
@@ -577,38 +568,86 @@ int orc_trace_frame(const orc_trace_input *in, const uint16_t *bases, uint32_t n
         break;
       }
       if (q == Q_BASE) {
-        q = base_step(&t, &r, &acc);
+        q = base_step(t, &r, &acc);
       } else {
-        q = block_step(&t, &r, &acc, &child, &req_tmp[id]);
+        q = block_step(t, &r, &acc, &child, &req_tmp[id]);
         if (child) req_flag[id] = 1;
       }
     }
     uint32_t hit;
     float rgba[4];
-    shade(&t, &r, q, &hit, rgba);
-    write_ray(out, id, &r, q, child, capped, &acc, hit, rgba);
+    shade(t, &r, q, &hit, rgba);
+    write_ray(out, (uint32_t)id, &r, q, child, capped, &acc, hit, rgba);
     base_total += acc.base_steps;
     block_total += acc.block_steps;
   }
-  uint32_t n_req = 0;
-  for (size_t i = 0; i < N; i++) {
-    if (!req_flag[i]) continue;
-    if (requests && n_req < request_capacity) requests[n_req] = req_tmp[i];
-    n_req++;
-  }
-  if (request_count) *request_count = n_req;
-  uint64_t groups = 0;
-  apply_usage(usages, n_usages, touched, in->tick, &groups);
-  if (out) {
-    out->base_steps = base_total;
-    out->block_steps = block_total;
-    out->groups_touched = groups;
-  }
+  *base_sum += base_total;
+  *block_sum += block_total;
   free(lut);
+  return 0;
+}
+
+/* Several cameras in one frame (the batched form): outputs concatenated camera by camera,
+ * child requests in ascending concatenated cone index, one usage tick for all. */
+int orc_trace_batch(const orc_trace_input *ins, const float *apertures, uint32_t n_cams, const uint16_t *bases,
+                    uint32_t n_bases, const orc_block *blocks, uint32_t n_blocks, orc_usage *usages,
+                    uint32_t n_usages, orc_child_request *requests, uint32_t request_capacity,
+                    uint32_t *request_count, orc_frame_out *out, int threads) {
+  if (!ins || n_cams == 0) return -1;
+  size_t N = 0;
+  for (uint32_t c = 0; c < n_cams; c++) {
+    int rc = check_input(&ins[c], n_bases);
+    if (rc) return rc;
+    if (ins[c].tick != ins[0].tick || ins[c].base_depth != ins[0].base_depth) return -4;
+    N += (size_t)ins[c].screen_size[0] * ins[c].screen_size[1];
+  }
+  uint8_t *touched = usages ? (uint8_t *)calloc(n_usages ? n_usages : 1, 1) : NULL;
+  uint8_t *req_flag = (uint8_t *)calloc(N ? N : 1, 1);
+  orc_child_request *req_tmp = (orc_child_request *)malloc(sizeof(orc_child_request) * (N ? N : 1));
+  if ((usages && !touched) || !req_flag || !req_tmp) {
+    free(touched);
+    free(req_flag);
+    free(req_tmp);
+    return -10;
+  }
+  tree_t t;
+  setup_tree(&t, &ins[0], bases, n_bases, blocks, n_blocks, usages, n_usages, touched);
+  uint64_t base_total = 0, block_total = 0;
+  size_t offset = 0;
+  int rc = 0;
+  for (uint32_t c = 0; c < n_cams && !rc; c++) {
+    rc = frame_core(&ins[c], apertures ? apertures[c] : 1.0f, &t, offset, req_flag, req_tmp, out, threads,
+                    &base_total, &block_total);
+    offset += (size_t)ins[c].screen_size[0] * ins[c].screen_size[1];
+  }
+  if (!rc) {
+    uint32_t n_req = 0;
+    for (size_t i = 0; i < N; i++) {
+      if (!req_flag[i]) continue;
+      if (requests && n_req < request_capacity) requests[n_req] = req_tmp[i];
+      n_req++;
+    }
+    if (request_count) *request_count = n_req;
+    uint64_t groups = 0;
+    apply_usage(usages, n_usages, touched, ins[0].tick, &groups);
+    if (out) {
+      out->base_steps = base_total;
+      out->block_steps = block_total;
+      out->groups_touched = groups;
+    }
+  }
   free(touched);
   free(req_flag);
   free(req_tmp);
-  return 0;
+  return rc;
+}
+
+int orc_trace_frame(const orc_trace_input *in, const uint16_t *bases, uint32_t n_bases,
+                    const orc_block *blocks, uint32_t n_blocks, orc_usage *usages, uint32_t n_usages,
+                    orc_child_request *requests, uint32_t request_capacity, uint32_t *request_count,
+                    orc_frame_out *out, int threads) {
+  return orc_trace_batch(in, NULL, 1, bases, n_bases, blocks, n_blocks, usages, n_usages, requests,
+                         request_capacity, request_count, out, threads);
 }
 
 int orc_trace_rays(const orc_trace_input *in, const uint16_t *bases, uint32_t n_bases,
@@ -634,7 +673,7 @@ int orc_trace_rays(const orc_trace_input *in, const uint16_t *bases, uint32_t n_
     acc_init(&acc, NULL, id);
     orc_child_request req;
     int child = 0;
-    int q = init_ray_lut(in, cu, su, cv, sv, x, y, &r) ? Q_BASE : Q_BACKGROUND;
+    int q = init_ray_lut(in, 1.0f, cu, su, cv, sv, x, y, &r) ? Q_BASE : Q_BACKGROUND;
     while (q == Q_BASE || q == Q_BLOCK) {
       if (acc.visits >= STEP_CAP) {
         r.depth = 0;
@@ -692,7 +731,7 @@ int orc_wavefront_frame(const orc_trace_input *in, const uint16_t *bases, uint32
   /* init */
   for (uint32_t id = 0; id < N; id++) {
     acc_init(&accs[id], out, id);
-    if (init_ray_lut(in, cu, su, cv, sv, id % W, id / W, &rays[id]))
+    if (init_ray_lut(in, 1.0f, cu, su, cv, sv, id % W, id / W, &rays[id]))
       base_q[n_base++] = id;
     else
       background_queue[n_bg++] = id;
@@ -790,4 +829,140 @@ int orc_wavefront_frame(const orc_trace_input *in, const uint16_t *bases, uint32
   if (iterations) *iterations = iters;
   free(lut), free(rays), free(accs), free(childf), free(cappedf), free(qa), free(qb), free(ka), free(kb), free(touched);
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* differentiable compositing (no reference counterpart)                     */
+/* ------------------------------------------------------------------------ */
+
+/* Continues a composited cone past the cell it stopped in: the empty-octant transition
+ * of runBlockTrace (wavefront.cl:1505-1514) at the cone's depth. */
+static int pass_cell(const tree_t *t, ray_t *r) {
+  uint32_t D = r->depth, B = t->B;
+  uint64_t prev[3] = {r->loc[0], r->loc[1], r->loc[2]};
+  orc_traverse_chunk(D, r->loc, r->dir, r->inv, r->sign);
+  if (leaving(r->loc, r->sign)) {
+    r->depth = 0;
+    return Q_BACKGROUND;
+  }
+  D = orc_compare_positions(D, prev, r->loc, r->sign);
+  if (D == B + 1) D = B;
+  r->depth = D;
+  return D > B ? Q_BLOCK : Q_BASE;
+}
+
+/* Sky or sun colour of a direction (evaluateBackground, wavefront.cl:1762-1772), in [0,1]. */
+static void background_colour(const float dir[3], float bg[3]) {
+  float dp = (dir[0] * 0.0f + dir[1] * 0.0f) + dir[2] * -1.0f;
+  if (dp > 0.8f) {
+    bg[0] = bg[1] = bg[2] = 1.0f;
+  } else {
+    bg[0] = 135.0f / 255.0f;
+    bg[1] = 206.0f / 255.0f;
+    bg[2] = 235.0f / 255.0f;
+  }
+}
+
+int orc_composite_layers(const orc_trace_input *ins, const float *apertures, uint32_t n_views, const uint16_t *bases,
+                         uint32_t n_bases, const orc_block *blocks, uint32_t n_blocks, uint32_t max_layers,
+                         uint32_t *layers, float *bg, int threads) {
+  if (!ins || n_views == 0 || max_layers == 0) return -1;
+  size_t offset = 0;
+  for (uint32_t v = 0; v < n_views; v++) {
+    const orc_trace_input *in = &ins[v];
+    int rc = check_input(in, n_bases);
+    if (rc) return rc;
+    const uint32_t W = in->screen_size[0], H = in->screen_size[1];
+    const size_t N = (size_t)W * H;
+    const float aperture = apertures ? apertures[v] : 1.0f;
+    float *lut = (float *)malloc(sizeof(float) * 2 * ((size_t)W + H));
+    if (!lut) return -10;
+    float *cu = lut, *su = lut + W, *cv = lut + 2 * W, *sv = lut + 2 * W + H;
+    orc_camera_lut(in, cu, su, cv, sv);
+    tree_t t;
+    setup_tree(&t, in, bases, n_bases, blocks, n_blocks, NULL, 0, NULL);
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+#endif
+    for (long long i = 0; i < (long long)N; i++) {
+      const size_t id = offset + (size_t)i;
+      uint32_t *L = layers + id * max_layers;
+      for (uint32_t k = 0; k < max_layers; k++) L[k] = U32MAX;
+      ray_t r;
+      acc_t acc;
+      acc_init(&acc, NULL, 0);
+      int q = init_ray_lut(in, aperture, cu, su, cv, sv, (uint32_t)i % W, (uint32_t)i / W, &r) ? Q_BASE : Q_BACKGROUND;
+      uint32_t n = 0;
+      int child = 0;
+      orc_child_request req;
+      while (q != Q_BACKGROUND && n < max_layers) {
+        if (acc.visits >= STEP_CAP) break;
+        if (q == Q_BASE)
+          q = base_step(&t, &r, &acc);
+        else if (q == Q_BLOCK)
+          q = block_step(&t, &r, &acc, &child, &req);
+        if (q == Q_MATERIAL) {
+          L[n++] = block_address(&t, r.depth, r.loc);
+          q = pass_cell(&t, &r);
+        }
+      }
+      background_colour(r.dir, bg + 3 * id);
+    }
+    free(lut);
+    offset += N;
+  }
+  return 0;
+}
+
+/* Front-to-back compositing of the layer lists: C = sum_i T_i a_i c_i + T_end bg with
+ * T_1 = 1, T_{i+1} = T_i (1 - a_i); params are (r, g, b, a) per block.  Returns the L2
+ * loss sum over rays and channels of (C - target)^2 over the listed rays (all when rays is
+ * NULL), optionally the image and the gradient (accumulated into grads, 4 per block):
+ * dC/dc_i = T_i a_i, dC/da_i = T_i (c_i - U_i) with U_i the composite behind layer i. */
+double orc_composite_loss(const uint32_t *layers, const float *bg, const uint32_t *rays, uint32_t n_rays,
+                          uint32_t max_layers, const double *params, uint32_t n_blocks, const double *target,
+                          double *image, double *grads) {
+  double loss = 0.0;
+  double T[64], U[3];
+  if (max_layers > 63) return -1.0;
+  for (uint32_t j = 0; j < n_rays; j++) {
+    const size_t id = rays ? rays[j] : j;
+    const uint32_t *L = layers + id * max_layers;
+    const float *b = bg + 3 * id;
+    double C[3] = {0.0, 0.0, 0.0};
+    uint32_t n = 0;
+    T[0] = 1.0;
+    for (; n < max_layers && L[n] != U32MAX; n++) {
+      const uint32_t a = L[n];
+      const double *p = a < n_blocks ? params + 4 * (size_t)a : NULL;
+      const double al = p ? p[3] : 0.0;
+      for (int c = 0; c < 3; c++) C[c] += T[n] * al * (p ? p[c] : 0.0);
+      T[n + 1] = T[n] * (1.0 - al);
+    }
+    for (int c = 0; c < 3; c++) C[c] += T[n] * (double)b[c];
+    double g[3];
+    for (int c = 0; c < 3; c++) {
+      const double e = C[c] - (target ? target[3 * id + c] : 0.0);
+      loss += e * e;
+      g[c] = 2.0 * e;
+      if (image) image[3 * id + c] = C[c];
+    }
+    if (!grads) continue;
+    for (int c = 0; c < 3; c++) U[c] = (double)b[c];
+    for (int i = (int)n - 1; i >= 0; i--) {
+      const uint32_t a = L[i];
+      if (a >= n_blocks) continue;
+      const double *p = params + 4 * (size_t)a;
+      double *gr = grads + 4 * (size_t)a;
+      double da = 0.0;
+      for (int c = 0; c < 3; c++) {
+        gr[c] += g[c] * T[i] * p[3];
+        da += g[c] * T[i] * (p[c] - U[c]);
+      }
+      gr[3] += da;
+      for (int c = 0; c < 3; c++) U[c] = p[3] * p[c] + (1.0 - p[3]) * U[c];
+    }
+  }
+  return loss;
 }

@@ -116,6 +116,14 @@ int orc_trace_frame(const orc_trace_input *in, const uint16_t *bases, uint32_t n
                     orc_child_request *requests, uint32_t request_capacity, uint32_t *request_count,
                     orc_frame_out *out, int threads);
 
+/* Several cameras in one frame: outputs concatenated camera by camera (arrays sized for the
+ * sum of W*H), cone aperture apertures[c] pixels (NULL: 1), child requests in ascending
+ * concatenated index; all cameras must share tick and base depth. */
+int orc_trace_batch(const orc_trace_input *ins, const float *apertures, uint32_t n_cams, const uint16_t *bases,
+                    uint32_t n_bases, const orc_block *blocks, uint32_t n_blocks, orc_usage *usages,
+                    uint32_t n_usages, orc_child_request *requests, uint32_t request_capacity,
+                    uint32_t *request_count, orc_frame_out *out, int threads);
+
 /* Trace only the rays listed in `ray_ids` (a bounded sample, used for the CPU baseline).
  * Usages are not touched.  hit/visits are written at the sample position. */
 int orc_trace_rays(const orc_trace_input *in, const uint16_t *bases, uint32_t n_bases,
@@ -130,6 +138,18 @@ int orc_wavefront_frame(const orc_trace_input *in, const uint16_t *bases, uint32
                         uint32_t *background_queue, uint32_t *background_count,
                         uint32_t *material_queue, uint32_t *material_count,
                         uint32_t *iterations, orc_frame_out *out);
+
+/* ---- differentiable compositing (no reference counterpart) ----
+ * A cone does not end at a material hit: the hit block (the block resolveRemainders would
+ * read) becomes a layer and the cone continues past the cell, up to max_layers layers or
+ * until it leaves the volume.  layers: max_layers u32 per cone (UINT32_MAX after the last);
+ * bg: the sky/sun colour of the cone's direction (3 floats in [0,1]). */
+int orc_composite_layers(const orc_trace_input *ins, const float *apertures, uint32_t n_views, const uint16_t *bases,
+                         uint32_t n_bases, const orc_block *blocks, uint32_t n_blocks, uint32_t max_layers,
+                         uint32_t *layers, float *bg, int threads);
+double orc_composite_loss(const uint32_t *layers, const float *bg, const uint32_t *rays, uint32_t n_rays,
+                          uint32_t max_layers, const double *params, uint32_t n_blocks, const double *target,
+                          double *image, double *grads);
 
 #ifdef __cplusplus
 }

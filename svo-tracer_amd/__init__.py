@@ -106,6 +106,12 @@ class BufferName(enum.IntEnum):
     VisitCounts = 66
     VisitHashes = 67
     RayStatus = 68
+    LeafParams = 69
+    TargetImage = 70
+    LeafGradients = 71
+    Layers = 72
+    AdamM = 73
+    AdamV = 74
 
 
 STATUS_NAMES = {
@@ -127,6 +133,7 @@ STATUS_NAMES = {
 FRAME_STATE = 1
 FRAME_QUEUES = 2
 FRAME_NO_XCD_SPLIT = 4
+FRAME_CAMERA_TABLE = 8
 
 GEOMETRY_SPHERES = 0
 GEOMETRY_CUBES = 1
@@ -216,6 +223,19 @@ class TreeStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class CompositeOptions(C.Structure):
+    _fields_ = [("max_layers", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class AdamOptions(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("step", C.c_uint32)]
+
+
+COMPOSITE_ADJOINT = 1
+COMPOSITE_LAYERS = 2
 
 
 class UpdateInputData(C.Structure):
@@ -321,6 +341,14 @@ _PROTOS = {
     "svo_finish": [C.c_void_p],
     "svo_trace_frame": [C.c_void_p, C.POINTER(TraceInput), C.POINTER(FrameOptions)],
     "svo_frame_stats_get": [C.c_void_p, C.POINTER(FrameStats)],
+    "svo_trace_batch": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(FrameOptions)],
+    "svo_frame_graph_create": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(FrameOptions),
+                               C.c_uint32, C.POINTER(C.c_void_p)],
+    "svo_frame_graph_launch": [C.c_void_p, C.c_void_p, C.POINTER(TraceInput)],
+    "svo_frame_graph_destroy": [C.c_void_p, C.c_void_p],
+    "svo_composite": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(CompositeOptions),
+                      C.POINTER(C.c_double)],
+    "svo_adam_step": [C.c_void_p, C.POINTER(AdamOptions)],
     "svo_timer_begin": [C.c_void_p],
     "svo_timer_end": [C.c_void_p, C.POINTER(C.c_float)],
     "svo_kernel_timing_enable": [C.c_void_p, C.c_int],
@@ -569,6 +597,58 @@ class ComputeManager:
         o.waves_per_simd = waves_per_simd
         _check(lib().svo_trace_frame(self._h, C.byref(trace_input), C.byref(o)), "TraceFrame")
 
+    def TraceBatch(self, trace_inputs, apertures=None, flags: int = 0, max_tree_depth: int = 0,
+                   workgroup_size: int = 0, refill_lanes: int = 0, stack_levels: int = 0, work_chunk: int = 0,
+                   waves_per_simd: int = 0) -> None:
+        """svo_trace_batch: several cameras in one persistent launch; per-cone buffers hold the
+        cameras' cones concatenated in camera order."""
+        n = len(trace_inputs)
+        arr = (TraceInput * n)(*trace_inputs)
+        ap = None if apertures is None else np.ascontiguousarray(apertures, dtype=np.float32)
+        o = FrameOptions()
+        o.flags = flags
+        o.max_tree_depth = max_tree_depth
+        o.workgroup_size = workgroup_size
+        o.refill_lanes = refill_lanes
+        o.stack_levels = stack_levels
+        o.work_chunk = work_chunk
+        o.waves_per_simd = waves_per_simd
+        _check(lib().svo_trace_batch(self._h, arr, None if ap is None else ap.ctypes.data, n, C.byref(o)),
+               "TraceBatch")
+
+    def CaptureFrame(self, trace_inputs, apertures=None, per_camera: bool = False, **options) -> "FrameGraph":
+        """svo_frame_graph_create: records the submission of a (batched) frame into a HIP graph."""
+        n = len(trace_inputs)
+        arr = (TraceInput * n)(*trace_inputs)
+        ap = None if apertures is None else np.ascontiguousarray(apertures, dtype=np.float32)
+        o = FrameOptions()
+        for k, v in options.items():
+            setattr(o, k, v)
+        h = C.c_void_p()
+        _check(lib().svo_frame_graph_create(self._h, arr, None if ap is None else ap.ctypes.data, n, C.byref(o),
+                                            1 if per_camera else 0, C.byref(h)), "CaptureFrame")
+        return FrameGraph(self, h, n)
+
+    def Composite(self, views, apertures=None, max_layers: int = 0, adjoint: bool = False,
+                  layers: bool = False) -> float:
+        """svo_composite: differentiable compositing of the views into OutputImage; returns the
+        L2 loss against TargetImage (0 when no target is bound)."""
+        n = len(views)
+        arr = (TraceInput * n)(*views)
+        ap = None if apertures is None else np.ascontiguousarray(apertures, dtype=np.float32)
+        o = CompositeOptions()
+        o.max_layers = max_layers
+        o.flags = (COMPOSITE_ADJOINT if adjoint else 0) | (COMPOSITE_LAYERS if layers else 0)
+        loss = C.c_double(0.0)
+        _check(lib().svo_composite(self._h, arr, None if ap is None else ap.ctypes.data, n, C.byref(o),
+                                   C.byref(loss)), "Composite")
+        return loss.value
+
+    def AdamStep(self, step: int, learning_rate: float = 0.01, beta1: float = 0.9, beta2: float = 0.999,
+                 epsilon: float = 1e-8) -> None:
+        o = AdamOptions(learning_rate, beta1, beta2, epsilon, step)
+        _check(lib().svo_adam_step(self._h, C.byref(o)), "AdamStep")
+
     def FrameStats(self) -> FrameStats:
         s = FrameStats()
         _check(lib().svo_frame_stats_get(self._h, C.byref(s)), "FrameStats")
@@ -590,6 +670,24 @@ class ComputeManager:
         n = C.c_uint64(0)
         _check(lib().svo_kernel_timing_get(self._h, C.byref(total), C.byref(n), 1 if reset else 0), "KernelTiming")
         return total.value, n.value
+
+
+class FrameGraph:
+    """A captured frame submission (svo_frame_graph_*): Launch(new cameras) replays it."""
+
+    def __init__(self, cm: "ComputeManager", handle: C.c_void_p, n: int):
+        self.cm, self.handle, self.n = cm, handle, n
+
+    def Launch(self, trace_inputs) -> None:
+        if len(trace_inputs) != self.n:
+            raise ValueError("a captured frame replays with the same number of cameras")
+        arr = (TraceInput * self.n)(*trace_inputs)
+        _check(lib().svo_frame_graph_launch(self.cm._h, self.handle, arr), "LaunchFrame")
+
+    def Dispose(self) -> None:
+        if self.handle:
+            lib().svo_frame_graph_destroy(self.cm._h, self.handle)
+            self.handle = None
 
 
 class Octree:

@@ -19,6 +19,41 @@ namespace svo {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
+// Concatenated camera tables of a batch (cached while the resolutions and FoVs repeat);
+// offsets[c] = first float of camera c's table.
+int update_batch_tables(Context *ctx, const svo_trace_input *ins, uint32_t n, std::vector<uint32_t> *offsets) {
+  std::vector<float> key;
+  key.reserve(4 * (size_t)n);
+  size_t total = 0;
+  offsets->resize(n);
+  for (uint32_t c = 0; c < n; c++) {
+    key.push_back((float)ins[c].screen_size[0]);
+    key.push_back((float)ins[c].screen_size[1]);
+    key.push_back(ins[c].fov[0]);
+    key.push_back(ins[c].fov[1]);
+    (*offsets)[c] = (uint32_t)total;
+    total += 2 * ((size_t)ins[c].screen_size[0] + ins[c].screen_size[1]);
+  }
+  if (ctx->batch_lut_valid && ctx->batch_lut_key.size() == key.size() &&
+      std::memcmp(ctx->batch_lut_key.data(), key.data(), key.size() * sizeof(float)) == 0)
+    return SVO_OK;
+  std::vector<float> lut(total);
+  for (uint32_t c = 0; c < n; c++) {
+    const uint32_t W = ins[c].screen_size[0], H = ins[c].screen_size[1];
+    float *t = lut.data() + (*offsets)[c];
+    camera_lut(ins[c], t, t + W, t + 2 * W, t + 2 * W + H);
+  }
+  int rc = ensure_scratch(ctx->scratch_batch_lut, lut.size() * sizeof(float));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(ctx->scratch_batch_lut->ptr, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice,
+                         ctx->stream),
+          "camera tables upload");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "camera tables sync");
+  ctx->batch_lut_key = key;
+  ctx->batch_lut_valid = true;
+  return SVO_OK;
+}
+
 int update_camera_table(Context *ctx, const svo_trace_input &in) {
   const uint32_t W = in.screen_size[0], H = in.screen_size[1];
   if (ctx->lut_valid && ctx->lut_w == W && ctx->lut_h == H && std::memcmp(&ctx->lut_fov0, &in.fov[0], 4) == 0 &&
@@ -48,8 +83,8 @@ int check_size(const FrameBuffers::B &b, size_t need, const char *what) {
   return SVO_OK;
 }
 
-int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, const svo_frame_options &o, bool *bases_in_lds,
-                  uint32_t *stack_levels, LaunchShape *shape) {
+int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, const svo_frame_options &o,
+                  bool *bases_in_lds, uint32_t *stack_levels, LaunchShape *shape) {
   int wg = o.workgroup_size ? (int)o.workgroup_size : 256;
   if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
     return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup_size must be 64, 128, 256 or 512");
@@ -61,6 +96,7 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, const svo_frame_op
   size_t bases_bytes = ((size_t)n_bases * 2 + 3) & ~(size_t)3;
   *bases_in_lds = bases_bytes <= 32 * 1024;
   if (!*bases_in_lds) bases_bytes = 0;
+  bases_bytes += extra_lds;
   while (levels > 1 && bases_bytes + (size_t)levels * wg * 4 > lds_limit) levels--;
   size_t lds = bases_bytes + (size_t)levels * wg * 4;
   if (lds > lds_limit) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup too large for LDS");
@@ -83,16 +119,62 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, const svo_frame_op
 }  // namespace
 
 int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb) {
-  const uint32_t W = in.screen_size[0], H = in.screen_size[1], B = in.base_depth;
-  if (W == 0 || H == 0 || (uint64_t)W * H > (1ull << 31))
-    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: screen size must be non-zero and at most 2^31 cones");
+  return trace_batch_impl(ctx, &in, nullptr, 1, o, fb);
+}
+
+// Fills the per-frame fields of the camera records (pose, cone aperture, tick); the layout
+// fields (sizes, tile and cone offsets, table offsets) are set by plan_batch.
+void fill_camera_poses(const svo_trace_input *ins, const float *apertures, uint32_t n_cams, CamRec *cams) {
+  for (uint32_t c = 0; c < n_cams; c++) {
+    CamRec &r = cams[c];
+    for (int k = 0; k < 9; k++) r.facing[k] = ins[c].facing[k];
+    for (int k = 0; k < 3; k++) r.origin[k] = ins[c].origin[k];
+    // setupInitialTrace: FoV = FoV[0] / ScreenSize.x; an aperture of k pixels scales it
+    const float a = apertures ? apertures[c] : 1.0f;
+    r.cone_fov = (a * ins[c].fov[0]) / (float)ins[c].screen_size[0];
+    r.dof[0] = ins[c].dof[0];
+    r.dof[1] = ins[c].dof[1];
+    r.pad[0] = ins[0].tick;
+  }
+}
+
+int validate_batch(const svo_trace_input *ins, const float *apertures, uint32_t n_cams, size_t *cones) {
+  if (!ins || n_cams == 0) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: no camera");
+  if (n_cams > kMaxBatchCameras)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: at most " + std::to_string(kMaxBatchCameras) + " cameras");
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < n_cams; c++) {
+    const uint32_t w = ins[c].screen_size[0], h = ins[c].screen_size[1];
+    if (w == 0 || h == 0) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: screen size must be non-zero");
+    if (ins[c].base_depth != ins[0].base_depth || ins[c].tick != ins[0].tick)
+      return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: the cameras of a batch share BaseDepth and Tick");
+    if (apertures && !(apertures[c] > 0.0f && apertures[c] <= 64.0f))
+      return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: aperture must be in (0, 64] pixels");
+    total += (uint64_t)w * h;
+  }
+  if (total > (1ull << 31)) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: at most 2^31 cones per frame");
+  const uint32_t B = ins[0].base_depth;
   if (B < 1 || B > 7) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: base_depth must be 1..7");
-  const size_t N = (size_t)W * H;
+  *cones = (size_t)total;
+  return SVO_OK;
+}
+
+// Everything of a frame but its submission: buffers checked, scratch sized, camera tables
+// uploaded, launch shape chosen.  Camera records (multi) are written to `host_cams` and
+// read by the kernel from `dev_cams`.
+int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
+               const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
+               const float *lut_dev, const uint32_t *lut_offsets, BatchPlan *plan) {
+  size_t N = 0;
+  int rc = validate_batch(ins, apertures, n_cams, &N);
+  if (rc) return rc;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  const svo_trace_input &in = ins[0];
+  const uint32_t B = in.base_depth;
+  const bool multi = n_cams > 1 || (o.flags & SVO_FRAME_CAMERA_TABLE) != 0;
   if (!fb.bases.ptr) return fail(SVO_ERR_UNKNOWN_BUFFER, "trace frame: buffer Bases not initialised");
   if (!fb.blocks.ptr) return fail(SVO_ERR_UNKNOWN_BUFFER, "trace frame: buffer Blocks not initialised");
   const uint32_t n_bases = pow_sum(B);
-  int rc;
   if ((rc = check_size(fb.bases, (size_t)n_bases * 2, "Bases"))) return rc;
   if ((rc = check_size(fb.request_id, 4, "ChildRequestID"))) return rc;
   if ((rc = check_size(fb.rgba, 16 * N, "OutputImage"))) return rc;
@@ -101,7 +183,7 @@ int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_op
   if ((rc = check_size(fb.hashes, 8 * N, "VisitHashes"))) return rc;
   if ((rc = check_size(fb.status, N, "RayStatus"))) return rc;
 
-  FrameParams p;
+  FrameParams &p = plan->p;
   std::memset(&p, 0, sizeof p);
   p.bases = (const uint16_t *)fb.bases.ptr;
   p.n_bases = n_bases;
@@ -126,7 +208,8 @@ int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_op
     p.loc_out = (uint64_t *)fb.locations.ptr;
     p.depth_out = (uint8_t *)fb.depths.ptr;
   }
-  if (o.flags & SVO_FRAME_QUEUES) {
+  const bool queues = (o.flags & SVO_FRAME_QUEUES) != 0;
+  if (queues) {
     if (!fb.bg_queue.ptr || !fb.mat_queue.ptr || !fb.bg_count.ptr || !fb.mat_count.ptr)
       return fail(SVO_ERR_UNKNOWN_BUFFER, "trace frame: SVO_FRAME_QUEUES needs Background/MaterialQueue and their IDs");
     if ((rc = check_size(fb.bg_queue, 4 * N, "BackgroundQueue"))) return rc;
@@ -134,35 +217,68 @@ int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_op
     if ((rc = check_size(fb.bg_count, 4, "BackgroundQueueID"))) return rc;
     if ((rc = check_size(fb.mat_count, 4, "MaterialQueueID"))) return rc;
   }
-  const bool queues = (o.flags & SVO_FRAME_QUEUES) != 0;
-  const bool compaction = requests || request_id || queues;
-  if (compaction && !p.status) {
-    if ((rc = ensure_scratch(ctx->scratch_status, N))) return rc;
-    p.status = (uint8_t *)ctx->scratch_status->ptr;
+  plan->compaction = requests || request_id || queues;
+  if (plan->compaction && !p.status) {
+    if ((rc = ensure_scratch(sc.status, N))) return rc;
+    p.status = (uint8_t *)sc.status->ptr;
   }
   if (requests || request_id) {
-    if ((rc = ensure_scratch(ctx->scratch_requests, N * sizeof(svo_child_request)))) return rc;
-    p.req_sparse = (svo_child_request *)ctx->scratch_requests->ptr;
+    if ((rc = ensure_scratch(sc.requests, N * sizeof(svo_child_request)))) return rc;
+    p.req_sparse = (svo_child_request *)sc.requests->ptr;
   }
-  if ((rc = ensure_scratch(ctx->scratch_small, kSmallScratchBytes))) return rc;
-  char *small = (char *)ctx->scratch_small->ptr;
+  if ((rc = ensure_scratch(sc.small, kSmallScratchBytes))) return rc;
+  char *small = (char *)sc.small->ptr;
   p.counters = (uint32_t *)(small + kCountersOffset);
   p.totals = (unsigned long long *)(small + kTotalsOffset);
-  if ((rc = update_camera_table(ctx, in))) return rc;
-  p.lut = (const float *)ctx->scratch_lut->ptr;
 
-  for (int k = 0; k < 9; k++) p.facing[k] = in.facing[k];
-  for (int k = 0; k < 3; k++) p.origin[k] = in.origin[k];
-  p.dof[0] = in.dof[0];
-  p.dof[1] = in.dof[1];
-  p.cone_fov = in.fov[0] / (float)W;  // setupInitialTrace: FoV = FoV[0] / ScreenSize.x
-  p.W = W;
-  p.H = H;
+  uint32_t tiles = 0;
+  if (!multi) {
+    if ((rc = update_camera_table(ctx, in))) return rc;
+    p.lut = (const float *)ctx->scratch_lut->ptr;
+    CamRec one;
+    std::memset(&one, 0, sizeof one);
+    fill_camera_poses(ins, apertures, 1, &one);
+    for (int k = 0; k < 9; k++) p.facing[k] = one.facing[k];
+    for (int k = 0; k < 3; k++) p.origin[k] = one.origin[k];
+    p.dof[0] = one.dof[0];
+    p.dof[1] = one.dof[1];
+    p.cone_fov = one.cone_fov;
+    p.W = in.screen_size[0];
+    p.H = in.screen_size[1];
+    p.tiles_x = (p.W + 7) / 8;
+    tiles = p.tiles_x * ((p.H + 7) / 8);
+  } else {
+    std::vector<uint32_t> offsets;
+    if (lut_dev) {
+      offsets.assign(lut_offsets, lut_offsets + n_cams);
+      p.lut = lut_dev;
+    } else {
+      if ((rc = update_batch_tables(ctx, ins, n_cams, &offsets))) return rc;
+      p.lut = (const float *)ctx->scratch_batch_lut->ptr;
+    }
+    uint64_t ray0 = 0;
+    for (uint32_t c = 0; c < n_cams; c++) {
+      CamRec &r = host_cams[c];
+      std::memset(&r, 0, sizeof r);
+      r.W = ins[c].screen_size[0];
+      r.H = ins[c].screen_size[1];
+      r.tiles_x = (r.W + 7) / 8;
+      r.tile_begin = tiles;
+      r.ray_begin = (uint32_t)ray0;
+      r.lut_offset = offsets[c];
+      tiles += r.tiles_x * ((r.H + 7) / 8);
+      ray0 += (uint64_t)r.W * r.H;
+    }
+    fill_camera_poses(ins, apertures, n_cams, host_cams);
+    p.cams = dev_cams;
+    p.n_cams = n_cams;
+  }
   p.B = B;
   p.tick = in.tick;
-  p.tiles_x = (W + 7) / 8;
-  const uint32_t tiles = p.tiles_x * ((H + 7) / 8);
-  for (int r = 0; r <= 8; r++) p.range_begin[r] = (uint32_t)(((uint64_t)tiles * r) / 8);
+  // eight bands of tiles, one per XCD (work-group b runs on XCD b mod 8); without the split
+  // every wave claims from one shared range
+  const bool split = (o.flags & SVO_FRAME_NO_XCD_SPLIT) == 0;
+  for (int r = 0; r <= 8; r++) p.range_begin[r] = split ? (uint32_t)(((uint64_t)tiles * r) / 8) : (r ? tiles : 0u);
   p.refill = o.refill_lanes ? std::min<uint32_t>(o.refill_lanes, 64u) : 64u;
   // defaults: a wave takes one 8x8 tile per claim and refills only when all its lanes are
   // idle -- spatial coherence of the 64 cones matters more than lane utilisation
@@ -170,35 +286,20 @@ int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_op
 
   bool bases_in_lds = false;
   uint32_t levels = 1;
-  LaunchShape shape;
-  if ((rc = choose_launch(ctx, B, n_bases, o, &bases_in_lds, &levels, &shape))) return rc;
+  if ((rc = choose_launch(ctx, B, n_bases, multi ? (size_t)n_cams * sizeof(CamRec) : 0, o, &bases_in_lds, &levels,
+                          &plan->shape)))
+    return rc;
   p.bases_in_lds = bases_in_lds ? 1u : 0u;
   p.stack_levels = levels;
 
-  if (launch_frame_prepare(p.counters, p.totals, ctx->stream) != 0)
-    return hip_fail(hipGetLastError(), "trace frame: prepare launch");
-  std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
-  if (ctx->kernel_timing) {
-    if (ctx->kernel_events_used == ctx->kernel_events.size()) {
-      std::pair<hipEvent_t, hipEvent_t> pr;
-      HIP_TRY(hipEventCreate(&pr.first), "hipEventCreate");
-      HIP_TRY(hipEventCreate(&pr.second), "hipEventCreate");
-      ctx->kernel_events.push_back(pr);
-    }
-    ev = &ctx->kernel_events[ctx->kernel_events_used++];
-    HIP_TRY(hipEventRecord(ev->first, ctx->stream), "hipEventRecord");
-  }
-  if (launch_trace_frame(p, shape, ctx->stream) != 0) return hip_fail(hipGetLastError(), "trace frame: trace launch");
-  if (ev) HIP_TRY(hipEventRecord(ev->second, ctx->stream), "hipEventRecord");
-
-  if (compaction) {
+  CompactParams &c = plan->compact;
+  std::memset(&c, 0, sizeof c);
+  if (plan->compaction) {
     const uint32_t chunks = compaction_chunks((uint32_t)N);
-    if ((rc = ensure_scratch(ctx->scratch_counts, (size_t)std::max(1u, chunks) * 16))) return rc;
-    CompactParams c;
-    std::memset(&c, 0, sizeof c);
+    if ((rc = ensure_scratch(sc.counts, (size_t)std::max(1u, chunks) * 16))) return rc;
     c.status = p.status;
     c.n = (uint32_t)N;
-    c.counts = (uint32_t *)ctx->scratch_counts->ptr;
+    c.counts = (uint32_t *)sc.counts->ptr;
     if (queues) {
       c.bg_queue = (uint32_t *)fb.bg_queue.ptr;
       c.mat_queue = (uint32_t *)fb.mat_queue.ptr;
@@ -210,9 +311,61 @@ int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_op
     c.requests = requests;
     c.req_capacity = req_capacity;
     c.totals = p.totals;
-    if (launch_compaction(c, ctx->stream) != 0) return hip_fail(hipGetLastError(), "trace frame: compaction launch");
   }
-  ctx->last_rays = N;
+  plan->cones = N;
+  plan->n_cams = multi ? n_cams : 0;
+  return SVO_OK;
+}
+
+// The stream work of one planned frame: camera records (multi), counter reset, trace
+// kernel, compaction.  No host synchronisation and no memset nodes (capturable).
+int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool timing) {
+  if (plan.n_cams) {
+    HIP_TRY(hipMemcpyAsync((void *)plan.p.cams, host_cams, plan.n_cams * sizeof(CamRec), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "camera upload");
+  }
+  if (launch_frame_prepare(plan.p.counters, plan.p.totals, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "trace frame: prepare launch");
+  std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+  if (timing && ctx->kernel_timing) {
+    if (ctx->kernel_events_used == ctx->kernel_events.size()) {
+      std::pair<hipEvent_t, hipEvent_t> pr;
+      HIP_TRY(hipEventCreate(&pr.first), "hipEventCreate");
+      HIP_TRY(hipEventCreate(&pr.second), "hipEventCreate");
+      ctx->kernel_events.push_back(pr);
+    }
+    ev = &ctx->kernel_events[ctx->kernel_events_used++];
+    HIP_TRY(hipEventRecord(ev->first, ctx->stream), "hipEventRecord");
+  }
+  if (launch_trace_frame(plan.p, plan.shape, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "trace frame: trace launch");
+  if (ev) HIP_TRY(hipEventRecord(ev->second, ctx->stream), "hipEventRecord");
+  if (plan.compaction && launch_compaction(plan.compact, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "trace frame: compaction launch");
+  ctx->last_totals = plan.p.totals;
+  return SVO_OK;
+}
+
+int trace_batch_impl(Context *ctx, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
+                     const svo_frame_options &o, const FrameBuffers &fb) {
+  if (!ctx->cam_host) {
+    HIP_TRY(hipHostMalloc((void **)&ctx->cam_host, kMaxBatchCameras * sizeof(CamRec), hipHostMallocDefault),
+            "hipHostMalloc");
+    HIP_TRY(hipEventCreateWithFlags(&ctx->cam_copied, hipEventDisableTiming), "hipEventCreate");
+  } else {
+    // the previous frame's camera upload must have read the staging records
+    HIP_TRY(hipEventSynchronize(ctx->cam_copied), "camera table reuse");
+  }
+  int rc = ensure_scratch(ctx->frame_scratch.cams, kMaxBatchCameras * sizeof(CamRec));
+  if (rc) return rc;
+  BatchPlan plan;
+  if ((rc = plan_batch(ctx, ctx->frame_scratch, ins, apertures, n_cams, o, fb, ctx->cam_host,
+                       (CamRec *)ctx->frame_scratch.cams->ptr, nullptr, nullptr, &plan)))
+    return rc;
+  if ((rc = submit_batch(ctx, plan, ctx->cam_host, true))) return rc;
+  HIP_TRY(hipEventRecord(ctx->cam_copied, ctx->stream), "hipEventRecord");
+  ctx->last_rays = plan.cones;
   return SVO_OK;
 }
 
@@ -253,4 +406,30 @@ extern "C" int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, c
   fb.bg_count = named(ctx, SVO_BUF_BACKGROUND_QUEUE_ID);
   fb.mat_count = named(ctx, SVO_BUF_MATERIAL_QUEUE_ID);
   return trace_frame_impl(ctx, *input, o, fb);
+}
+
+extern "C" int svo_trace_batch(svo_context *ctx, const svo_trace_input *inputs, const float *apertures,
+                               uint32_t n_cameras, const svo_frame_options *options) {
+  if (!ctx || !inputs) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_batch: null argument");
+  svo_frame_options o;
+  std::memset(&o, 0, sizeof o);
+  if (options) o = *options;
+  FrameBuffers fb;
+  fb.bases = named(ctx, SVO_BUF_BASES);
+  fb.blocks = named(ctx, SVO_BUF_BLOCKS);
+  fb.usages = named(ctx, SVO_BUF_USAGES);
+  fb.requests = named(ctx, SVO_BUF_CHILD_REQUESTS);
+  fb.request_id = named(ctx, SVO_BUF_CHILD_REQUEST_ID);
+  fb.rgba = named(ctx, SVO_BUF_OUTPUT_IMAGE);
+  fb.hit = named(ctx, SVO_BUF_HIT_INDICES);
+  fb.visits = named(ctx, SVO_BUF_VISIT_COUNTS);
+  fb.hashes = named(ctx, SVO_BUF_VISIT_HASHES);
+  fb.status = named(ctx, SVO_BUF_RAY_STATUS);
+  fb.locations = named(ctx, SVO_BUF_LOCATIONS);
+  fb.depths = named(ctx, SVO_BUF_DEPTHS);
+  fb.bg_queue = named(ctx, SVO_BUF_BACKGROUND_QUEUE);
+  fb.mat_queue = named(ctx, SVO_BUF_MATERIAL_QUEUE);
+  fb.bg_count = named(ctx, SVO_BUF_BACKGROUND_QUEUE_ID);
+  fb.mat_count = named(ctx, SVO_BUF_MATERIAL_QUEUE_ID);
+  return trace_batch_impl(ctx, inputs, apertures, n_cameras, o, fb);
 }

@@ -99,7 +99,7 @@ const std::vector<std::string> &kernel_params(int kernel) {
 }
 
 bool valid_buffer_name(int name) {
-  return (name >= 0 && name < SVO_BUF_REFERENCE_COUNT) || (name >= SVO_BUF_OUTPUT_IMAGE && name <= SVO_BUF_RAY_STATUS);
+  return (name >= 0 && name < SVO_BUF_REFERENCE_COUNT) || (name >= SVO_BUF_OUTPUT_IMAGE && name <= SVO_BUF_ADAM_V);
 }
 
 std::shared_ptr<DeviceBuffer> Context::get(int name) const {
@@ -159,10 +159,7 @@ int svo_destroy(svo_context *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->buffers.clear();
   ctx->args.clear();
-  ctx->scratch_status.reset();
-  ctx->scratch_requests.reset();
-  ctx->scratch_small.reset();
-  ctx->scratch_counts.reset();
+  ctx->frame_scratch = FrameScratch();
   ctx->scratch_lut.reset();
   for (auto &pr : ctx->kernel_events) {
     (void)hipEventDestroy(pr.first);
@@ -172,6 +169,9 @@ int svo_destroy(svo_context *ctx) {
   (void)hipEventDestroy(ctx->timer_end);
   for (auto &b : ctx->tree_scratch) b.reset();
   if (ctx->tree_host) (void)hipHostFree(ctx->tree_host);
+  if (ctx->cam_copied) (void)hipEventSynchronize(ctx->cam_copied);
+  if (ctx->cam_host) (void)hipHostFree(ctx->cam_host);
+  if (ctx->cam_copied) (void)hipEventDestroy(ctx->cam_copied);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SVO_OK;
@@ -405,9 +405,9 @@ int svo_kernel_timing_get(svo_context *ctx, double *total_ms, uint64_t *launches
 int svo_frame_stats_get(svo_context *ctx, svo_frame_stats *stats) {
   CHECK_CTX(ctx, "svo_frame_stats_get");
   if (!stats) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_frame_stats_get: null argument");
-  if (!ctx->scratch_small) return fail(SVO_ERR_NOT_SUPPORTED, "svo_frame_stats_get: no frame traced yet");
+  if (!ctx->last_totals) return fail(SVO_ERR_NOT_SUPPORTED, "svo_frame_stats_get: no frame traced yet");
   unsigned long long t[4] = {0, 0, 0, 0};
-  HIP_TRY(hipMemcpyAsync(t, (char *)ctx->scratch_small->ptr + kTotalsOffset, sizeof t, hipMemcpyDeviceToHost, ctx->stream),
+  HIP_TRY(hipMemcpyAsync(t, ctx->last_totals, sizeof t, hipMemcpyDeviceToHost, ctx->stream),
           "svo_frame_stats_get");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "svo_frame_stats_get: sync");
   stats->base_steps = t[0];

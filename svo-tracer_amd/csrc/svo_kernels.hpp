@@ -12,6 +12,19 @@ namespace svo {
 // work counters sit on separate 128-byte lines (u32 stride)
 constexpr uint32_t kCounterStride = 32;
 
+// One camera of a batched frame (96 bytes; read into LDS by the trace kernel).
+struct CamRec {
+  float facing[9];
+  float origin[3];
+  float cone_fov;
+  float dof[2];
+  uint32_t W, H, tiles_x, tile_begin, ray_begin, lut_offset;
+  uint32_t pad[3];
+};
+static_assert(sizeof(CamRec) == 96, "CamRec is 24 words");
+constexpr uint32_t kCamWords = 24;
+constexpr uint32_t kMaxBatchCameras = 256;
+
 // Everything one primary frame needs; passed by value as the kernel argument.
 struct FrameParams {
   // tree
@@ -43,6 +56,17 @@ struct FrameParams {
   uint8_t *depth_out;
   uint64_t *loc_out;  // ulong3 with a 32-byte stride
   svo_child_request *req_sparse;
+  // batched cameras (n_cams > 0: camera parameters come from this table, tiles of all
+  // cameras form one work space, outputs are indexed by the concatenated cone index)
+  const CamRec *cams;
+  uint32_t n_cams;
+  // differentiable compositing (composite modes only)
+  const float4 *leaf_params;  // (r, g, b, opacity) per block
+  const float4 *target;       // per cone, may be null
+  float4 *grads;              // per block (adjoint)
+  uint32_t *layers_out;       // max_layers per cone, may be null
+  double *loss;               // sum of squared errors, may be null
+  uint32_t max_layers;        // 1..4
   // tuning
   uint32_t stack_levels;
   uint32_t refill;
@@ -70,6 +94,11 @@ struct CompactParams {
 int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu);
 int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream);
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream);
+// Compositing modes of the trace kernel (camera table, 256-lane work-groups).
+enum { kModePlain = 0, kModeComposite = 1, kModeAdjoint = 2 };
+constexpr uint32_t kMaxLayers = 4;
+int launch_composite(const FrameParams &p, const LaunchShape &shape, int mode, hipStream_t stream);
+int composite_occupancy(size_t lds_bytes, int *blocks_per_cu);
 uint32_t compaction_chunks(uint32_t n);
 int launch_compaction(const CompactParams &c, hipStream_t stream);
 

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "svo.h"
+#include "svo_kernels.hpp"
 
 namespace svo {
 
@@ -55,6 +56,29 @@ int update_camera_table(Context *ctx, const svo_trace_input &in);
 // KernelName.Prune / KernelName.Graft behind svo_enqueue (arguments bound with SetArg).
 int enqueue_tree(Context *ctx, int kernel, uint32_t n);
 int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb);
+// A frame prepared for submission (svo_trace_frame / svo_trace_batch / captured graphs).
+struct BatchPlan {
+  FrameParams p;
+  LaunchShape shape;
+  CompactParams compact;
+  bool compaction = false;
+  size_t cones = 0;
+  uint32_t n_cams = 0;  // camera records uploaded before the kernel (0: single-camera kernel)
+};
+// Device scratch of frames: one set per context for direct calls, one per captured graph
+// (a graph keeps its own so later frames never move memory a replay writes to).
+struct FrameScratch {
+  std::shared_ptr<DeviceBuffer> status, requests, small, counts, cams;
+};
+int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
+               const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
+               const float *lut_dev, const uint32_t *lut_offsets, BatchPlan *plan);
+int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool timing);
+int validate_batch(const svo_trace_input *ins, const float *apertures, uint32_t n_cams, size_t *cones);
+void fill_camera_poses(const svo_trace_input *ins, const float *apertures, uint32_t n_cams, CamRec *cams);
+// Several cameras in one persistent launch; outputs concatenated camera by camera.
+int trace_batch_impl(Context *ctx, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
+                     const svo_frame_options &o, const FrameBuffers &fb);
 
 struct Context {
   int device = 0;
@@ -66,18 +90,26 @@ struct Context {
   uint32_t tree_max_depth = 0;
 
   // scratch owned by the context
-  std::shared_ptr<DeviceBuffer> scratch_status, scratch_requests, scratch_small, scratch_counts, scratch_lut;
+  FrameScratch frame_scratch;
+  std::shared_ptr<DeviceBuffer> scratch_lut;
   // scratch of the kernel-by-kernel path
   std::shared_ptr<DeviceBuffer> wave_codes, wave_values, wave_staging, wave_counts;
   // scratch of tree management (prune / graft / collect) and its pinned counter mirror
   std::shared_ptr<DeviceBuffer> tree_scratch[24];
   uint32_t *tree_host = nullptr;
+  // batched cameras: concatenated camera tables (cache key), camera records
+  std::shared_ptr<DeviceBuffer> scratch_batch_lut;
+  std::vector<float> batch_lut_key;
+  bool batch_lut_valid = false;
+  CamRec *cam_host = nullptr;  // pinned staging of the camera records
+  hipEvent_t cam_copied = nullptr;
   // camera table cache key
   uint32_t lut_w = 0, lut_h = 0;
   float lut_fov0 = 0.f, lut_fov1 = 0.f;
   bool lut_valid = false;
 
   uint64_t last_rays = 0;
+  const void *last_totals = nullptr;  // frame totals of the last submitted frame (device)
 
   // timing
   hipEvent_t timer_begin = nullptr, timer_end = nullptr;

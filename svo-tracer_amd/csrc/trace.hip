@@ -37,22 +37,39 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 // WAVES > 0 asks the compiler for that many waves per SIMD (a register budget).
 // HASH: the visit-sequence hash is computed only when its output is bound.
-template <int WG, int WAVES, bool HASH>
+// MULTI: cameras come from p.cams (staged in LDS); otherwise from the kernel argument.
+// MODE: kModePlain ends a cone at its first material hit (the reference's frame);
+// kModeComposite makes every hit a layer composited front to back and continues past the
+// cell (up to p.max_layers layers); kModeAdjoint also back-propagates the L2 image loss to
+// the layers' colour and opacity.
+template <int WG, int WAVES, bool HASH, bool MULTI, int MODE = kModePlain>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
 trace_frame_kernel(const FrameParams p) {
   extern __shared__ uint32_t smem[];
   uint16_t *s_bases = reinterpret_cast<uint16_t *>(smem);
   const uint32_t bases_words = p.bases_in_lds ? (p.n_bases + 1u) / 2u : 0u;
-  uint32_t *s_stack = smem + bases_words;
+  uint32_t *s_cam = smem + bases_words;
+  const uint32_t cam_words = MULTI ? p.n_cams * kCamWords : 0u;
+  uint32_t *s_stack = s_cam + cam_words;
   if (p.bases_in_lds) {
     for (uint32_t i = threadIdx.x; i < p.n_bases; i += WG) s_bases[i] = p.bases[i];
+  }
+  if (MULTI) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(p.cams);
+    for (uint32_t i = threadIdx.x; i < cam_words; i += WG) s_cam[i] = src[i];
   }
   __syncthreads();
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t B = p.B, S = p.stack_levels;
   const uint32_t xcd = blockIdx.x & 7u;
-  const float ox = p.origin[0], oy = p.origin[1], oz = p.origin[2];
+  // camera parameters: word w of camera c (MULTI) or of the single camera
+  auto camf = [&](uint32_t c, uint32_t w) -> float { return __uint_as_float(s_cam[c * kCamWords + w]); };
+  auto camu = [&](uint32_t c, uint32_t w) -> uint32_t { return s_cam[c * kCamWords + w]; };
+  uint32_t cam = 0;  // the lane's camera (MULTI)
+  // the frame tick: with a camera table it travels with the records (word 21 of camera 0),
+  // so a captured launch can be replayed for later frames
+  const uint32_t tick = MULTI ? camu(0, 21) : p.tick;
 
   uint32_t range_step = 0;  // wave-uniform: ranges tried, in order xcd, xcd+1, ...
   bool fetching = true;     // wave-uniform
@@ -68,8 +85,8 @@ trace_frame_kernel(const FrameParams p) {
   // frame every write stores the frame tick, so the deferral does not change the result
   uint32_t pend_g = kU32Max, pend_t = 0;
   auto resolve_usage = [&]() {
-    if (pend_g != kU32Max && pend_t < 65534u && pend_t != (p.tick & 0xffffu))
-      p.usage16[4u * (size_t)pend_g] = (uint16_t)p.tick;
+    if (pend_g != kU32Max && pend_t < 65534u && pend_t != (tick & 0xffffu))
+      p.usage16[4u * (size_t)pend_g] = (uint16_t)tick;
     pend_g = kU32Max;
   };
 
@@ -126,7 +143,7 @@ trace_frame_kernel(const FrameParams p) {
     if (child && p.req_sparse) {
       svo_child_request rq;
       rq.address = addr;
-      rq.tick = (uint16_t)p.tick;
+      rq.tick = (uint16_t)tick;
       rq.depth = (uint8_t)depth;
       rq.pad0 = 0;
       rq.location[0] = lx;
@@ -160,6 +177,76 @@ trace_frame_kernel(const FrameParams p) {
     }
   };
 
+  // ---- compositing state (composite modes) ----
+  float cT = 1.0f, cR = 0.0f, cG = 0.0f, cB = 0.0f;  // transmittance, colour so far
+  uint32_t nlay = 0, lay0 = kU32Max, lay1 = kU32Max, lay2 = kU32Max, lay3 = kU32Max;
+  double loss_acc = 0.0;
+  auto leaf = [&](uint32_t a) -> float4 {
+    return a < p.n_blocks ? p.leaf_params[a] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  };
+  auto layer_at = [&](uint32_t i) -> uint32_t { return i == 0 ? lay0 : i == 1 ? lay1 : i == 2 ? lay2 : lay3; };
+  // one layer: C += T a c, T *= 1 - a
+  auto composite = [&](uint32_t a) {
+    const float4 q = leaf(a);
+    const float w = cT * q.w;
+    cR += w * q.x;
+    cG += w * q.y;
+    cB += w * q.z;
+    cT = cT * (1.0f - q.w);
+    if (nlay == 0) lay0 = a;
+    else if (nlay == 1) lay1 = a;
+    else if (nlay == 2) lay2 = a;
+    else lay3 = a;
+    nlay++;
+  };
+  // The cone ends: the background takes the remaining transmittance; image, layers, loss
+  // and (adjoint) the gradient of every layer.
+  auto end_composite = [&]() {
+    const float dp = (dx * 0.0f + dy * 0.0f) + dz * -1.0f;
+    const bool sun = dp > 0.8f;
+    const float bg0 = sun ? 1.0f : 135.0f / 255.0f, bg1 = sun ? 1.0f : 206.0f / 255.0f,
+                bg2 = sun ? 1.0f : 235.0f / 255.0f;
+    const float C0 = cR + cT * bg0, C1 = cG + cT * bg1, C2 = cB + cT * bg2;
+    if (p.rgba) p.rgba[ray] = make_float4(C0, C1, C2, 1.0f);
+    if (p.layers_out)
+      for (uint32_t i = 0; i < p.max_layers; i++) p.layers_out[(size_t)ray * p.max_layers + i] = i < nlay ? layer_at(i) : kU32Max;
+    if (p.target) {
+      const float4 t = p.target[ray];
+      const float e0 = C0 - t.x, e1 = C1 - t.y, e2 = C2 - t.z;
+      loss_acc += (double)e0 * e0 + (double)e1 * e1 + (double)e2 * e2;
+      if (MODE == kModeAdjoint) {
+        const float g0 = 2.0f * e0, g1 = 2.0f * e1, g2 = 2.0f * e2;
+        // transmittance in front of each layer, then the composite behind it (back to front)
+        float Ti[kMaxLayers];
+        float T = 1.0f;
+        for (uint32_t i = 0; i < kMaxLayers; i++) {
+          Ti[i] = T;
+          if (i < nlay) T = T * (1.0f - leaf(layer_at(i)).w);
+        }
+        float U0 = bg0, U1 = bg1, U2 = bg2;
+        for (int i = (int)kMaxLayers - 1; i >= 0; i--) {
+          if ((uint32_t)i >= nlay) continue;
+          const uint32_t a = layer_at((uint32_t)i);
+          const float4 q = leaf(a);
+          if (a < p.n_blocks) {
+            float *gr = reinterpret_cast<float *>(p.grads + a);
+            const float w = Ti[i] * q.w;
+            atomicAdd(gr + 0, g0 * w);
+            atomicAdd(gr + 1, g1 * w);
+            atomicAdd(gr + 2, g2 * w);
+            atomicAdd(gr + 3, Ti[i] * ((g0 * (q.x - U0) + g1 * (q.y - U1)) + g2 * (q.z - U2)));
+          }
+          U0 = q.w * q.x + (1.0f - q.w) * U0;
+          U1 = q.w * q.y + (1.0f - q.w) * U1;
+          U2 = q.w * q.z + (1.0f - q.w) * U2;
+        }
+      }
+    }
+    tot_vis += nvis;
+    tot_blk += nblk;
+    active = false;
+  };
+
   for (;;) {
     // ---- refill idle lanes with new cones (wave-uniform control flow) ----
     const uint64_t idle = __ballot(!active);
@@ -188,18 +275,43 @@ trace_frame_kernel(const FrameParams p) {
         const uint32_t take = min(n_idle, cur_end - cur);
         cur += take;
         if (!active && k < cur_end) {
-          const uint32_t tile = cur_t0 + (k >> 6), l = k & 63u;
-          const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+          uint32_t tile = cur_t0 + (k >> 6);
+          const uint32_t l = k & 63u;
+          uint32_t W = p.W, H = p.H, tiles_x = p.tiles_x, ray0 = 0, lut0 = 0;
+          float f[9];
+          if (MULTI) {
+            // the camera whose tiles hold this one: the last with tile_begin <= tile
+            uint32_t lo = 0, hi = p.n_cams;
+            while (hi - lo > 1u) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (camu(mid, 18) <= tile)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            cam = lo;
+            tile -= camu(cam, 18);
+            W = camu(cam, 15);
+            H = camu(cam, 16);
+            tiles_x = camu(cam, 17);
+            ray0 = camu(cam, 19);
+            lut0 = camu(cam, 20);
+            for (int j = 0; j < 9; j++) f[j] = camf(cam, j);
+          } else {
+            for (int j = 0; j < 9; j++) f[j] = p.facing[j];
+          }
+          const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
           const uint32_t x = tx * 8u + (l & 7u), y = ty * 8u + (l >> 3);
-          if (x < p.W && y < p.H) {
+          if (x < W && y < H) {
             // setupInitialTrace, wavefront.cl:563-595 (trig from the camera table)
-            ray = x + y * p.W;
-            const float cu = p.lut[x], su = p.lut[p.W + x];
-            const float cv = p.lut[2u * p.W + y], sv = p.lut[2u * p.W + p.H + y];
+            ray = ray0 + x + y * W;
+            const float *lut = p.lut + lut0;
+            const float cu = lut[x], su = lut[W + x];
+            const float cv = lut[2u * W + y], sv = lut[2u * W + H + y];
             const float m0 = cu * cv, m1 = (0.0f - su) * cv, m2 = sv;
-            dx = (p.facing[0] * m0 + p.facing[3] * m1) + p.facing[6] * m2;
-            dy = (p.facing[1] * m0 + p.facing[4] * m1) + p.facing[7] * m2;
-            dz = (p.facing[2] * m0 + p.facing[5] * m1) + p.facing[8] * m2;
+            dx = (f[0] * m0 + f[3] * m1) + f[6] * m2;
+            dy = (f[1] * m0 + f[4] * m1) + f[7] * m2;
+            dz = (f[2] * m0 + f[5] * m1) + f[8] * m2;
             ix = 1.0f / dx;
             iy = 1.0f / dy;
             iz = 1.0f / dz;
@@ -208,14 +320,24 @@ trace_frame_kernel(const FrameParams p) {
             nblk = 0;
             hash = kFnvOffset;
             last_group = kU32Max;
-            const float o[3] = {ox, oy, oz}, d[3] = {dx, dy, dz}, inv[3] = {ix, iy, iz};
-            if (trace_into_volume(o, d, inv, p.cone_fov, lx, ly, lz)) {
+            if (MODE != kModePlain) {
+              cT = 1.0f;
+              cR = cG = cB = 0.0f;
+              nlay = 0;
+            }
+            const float o[3] = {MULTI ? camf(cam, 9) : p.origin[0], MULTI ? camf(cam, 10) : p.origin[1],
+                                MULTI ? camf(cam, 11) : p.origin[2]};
+            const float d[3] = {dx, dy, dz}, inv[3] = {ix, iy, iz};
+            if (trace_into_volume(o, d, inv, MULTI ? camf(cam, 12) : p.cone_fov, lx, ly, lz)) {
               depth = 1;
               active = true;
             } else {
               depth = 0;
               lx = ly = lz = 0;
-              retire(false, kU32Max, false, false, false);
+              if (MODE == kModePlain)
+                retire(false, kU32Max, false, false, false);
+              else
+                end_composite();
             }
           }
         }
@@ -230,7 +352,10 @@ trace_frame_kernel(const FrameParams p) {
     // ---- one step of the reference state machine ----
     if (nvis >= kStepCap) {
       depth = 0;
-      retire(false, kU32Max, false, false, true);
+      if (MODE == kModePlain)
+        retire(false, kU32Max, false, false, true);
+      else
+        end_composite();
       continue;
     }
     const uint32_t d = depth;
@@ -259,58 +384,85 @@ trace_frame_kernel(const FrameParams p) {
     nvis++;
     nblk += is_block ? 1u : 0u;
 
-    if (((chunk >> (2u * oct)) & 2u) == 0u) {
-      // empty octant: advance to the next cell boundary, pop levels that were left
+    // empty octant, or (composite modes) a hit that the cone continues past
+    bool advance = ((chunk >> (2u * oct)) & 2u) == 0u;
+    uint32_t dd = d;  // the depth the advance leaves from
+    if (!advance) {
+      if (!is_block) {
+        if (d < B) {
+          depth = d + 1u;
+        } else {
+          depth = B + 2u;
+          node_addr = morton_top(B + 2u, lx, ly, lz);
+          stack_put(B + 2u, node_addr);
+          node = load_block(p.blocks, p.n_blocks, node_addr);
+        }
+      } else {
+        // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
+        const float cone = MULTI ? cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), lx, ly, lz, camf(cam, 12),
+                                           camf(cam, 13), camf(cam, 14))
+                                 : cone_at(p.origin[0], p.origin[1], p.origin[2], lx, ly, lz, p.cone_fov, p.dof[0],
+                                           p.dof[1]);
+        bool hit = false, child = false, tree_size = false;
+        if (cone > exp2_neg(B + 2u)) {
+          depth = B + 2u;
+          node_addr = stack_get(B + 2u);
+          node = load_block(p.blocks, p.n_blocks, node_addr);
+          hit = true;
+        } else if (cone > exp2_neg(d)) {
+          depth = d - 1u;
+          node_addr = stack_get(d - 1u);
+          node = load_block(p.blocks, p.n_blocks, node_addr);
+        } else if (block_child(node) == kU32Max) {
+          // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
+          hit = child = true;
+          tree_size = cone <= exp2_neg(d + 1u);
+        } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
+          const uint32_t a = block_child(node) + oct;
+          depth = d + 1u;
+          stack_put(d + 1u, a);
+          node_addr = a;
+          node = load_block(p.blocks, p.n_blocks, a);
+        } else {
+          hit = true;
+        }
+        if (hit) {
+          if (MODE == kModePlain) {
+            retire(true, node_addr, child, tree_size, false);
+          } else {
+            composite(node_addr);
+            if (nlay >= p.max_layers) {
+              end_composite();
+            } else {
+              advance = true;
+              dd = depth;
+            }
+          }
+        }
+      }
+    }
+    if (advance) {
+      // advance to the next cell boundary, pop levels that were left
       const uint64_t px = lx, py = ly, pz = lz;
-      traverse_chunk(d, lx, ly, lz, dx, dy, dz, ix, iy, iz, signs);
+      traverse_chunk(dd, lx, ly, lz, dx, dy, dz, ix, iy, iz, signs);
       if (leaving(lx, ly, lz, signs)) {
         depth = 0;
-        retire(false, kU32Max, false, false, false);
+        if (MODE == kModePlain)
+          retire(false, kU32Max, false, false, false);
+        else
+          end_composite();
         continue;
       }
-      uint32_t nd = compare_positions(d, px, py, pz, lx, ly, lz, signs);
-      if (is_block) {
+      uint32_t nd = compare_positions(dd, px, py, pz, lx, ly, lz, signs);
+      if (dd > B) {
         if (nd == B + 1u) {
           nd = B;
-        } else if (nd > B + 1u && nd < d) {
+        } else if (nd > B + 1u && nd < dd) {
           node_addr = stack_get(nd);
           node = load_block(p.blocks, p.n_blocks, node_addr);
         }
       }
       depth = nd;
-    } else if (!is_block) {
-      if (d < B) {
-        depth = d + 1u;
-      } else {
-        depth = B + 2u;
-        node_addr = morton_top(B + 2u, lx, ly, lz);
-        stack_put(B + 2u, node_addr);
-        node = load_block(p.blocks, p.n_blocks, node_addr);
-      }
-    } else {
-      // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
-      const float cone = cone_at(ox, oy, oz, lx, ly, lz, p.cone_fov, p.dof[0], p.dof[1]);
-      if (cone > exp2_neg(B + 2u)) {
-        depth = B + 2u;
-        const uint32_t a = stack_get(B + 2u);
-        node = load_block(p.blocks, p.n_blocks, a);
-        retire(true, a, false, false, false);
-      } else if (cone > exp2_neg(d)) {
-        depth = d - 1u;
-        node_addr = stack_get(d - 1u);
-        node = load_block(p.blocks, p.n_blocks, node_addr);
-      } else if (block_child(node) == kU32Max) {
-        // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
-        retire(true, node_addr, true, cone <= exp2_neg(d + 1u), false);
-      } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
-        const uint32_t a = block_child(node) + oct;
-        depth = d + 1u;
-        stack_put(d + 1u, a);
-        node_addr = a;
-        node = load_block(p.blocks, p.n_blocks, a);
-      } else {
-        retire(true, node_addr, false, false, false);
-      }
     }
   }
 
@@ -321,6 +473,10 @@ trace_frame_kernel(const FrameParams p) {
   for (int off = 32; off > 0; off >>= 1) {
     vb += __shfl_xor(vb, off);
     vk += __shfl_xor(vk, off);
+  }
+  if (MODE != kModePlain && p.loss) {
+    for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
+    if (lane == 0) atomicAdd(p.loss, loss_acc);
   }
   if (lane == 0) {
     atomicAdd(&p.totals[0], (unsigned long long)vb);
@@ -466,21 +622,32 @@ __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
 
 template <int WG, int WAVES>
 int launch_trace_wg(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
-  if (p.hashes)
-    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
+  if (p.n_cams && p.hashes)
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
+                       stream, p);
+  else if (p.n_cams)
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
+                       stream, p);
+  else if (p.hashes)
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
+                       stream, p);
   else
-    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
+                       stream, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// Occupancy of the hashing variant: it never holds fewer registers than the plain one,
-// so a grid sized by it fits both.
+// Occupancy of the variant with the most registers bounds the grid of every variant.
 template <int WG, int WAVES>
 int occupancy_wg(size_t lds, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_frame_kernel<WG, WAVES, true>, WG, lds) ==
-                 hipSuccess
-             ? 0
-             : -1;
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, trace_frame_kernel<WG, WAVES, true, false>, WG, lds) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_frame_kernel<WG, WAVES, true, true>, WG, lds) !=
+          hipSuccess)
+    return -1;
+  *blocks_per_cu = a < b ? a : b;
+  return 0;
 }
 
 template <int WAVES>
@@ -524,6 +691,32 @@ int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream
     case 6: return launch_waves<6>(p, shape, stream);
     default: return -1;
   }
+}
+
+// The compositing modes hold more live state per lane (layers, colour, transmittance):
+// they are compiled for 4 waves per SIMD.
+int composite_occupancy(size_t lds_bytes, int *blocks_per_cu) {
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, trace_frame_kernel<256, 4, false, true, kModeComposite>, 256,
+                                                   lds_bytes) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_frame_kernel<256, 4, false, true, kModeAdjoint>, 256,
+                                                   lds_bytes) != hipSuccess)
+    return -1;
+  *blocks_per_cu = a < b ? a : b;
+  return 0;
+}
+
+int launch_composite(const FrameParams &p, const LaunchShape &shape, int mode, hipStream_t stream) {
+  if (shape.workgroup != 256 || !p.n_cams) return -1;
+  if (mode == kModeComposite)
+    hipLaunchKernelGGL((trace_frame_kernel<256, 4, false, true, kModeComposite>), dim3(shape.grid), dim3(256),
+                       shape.lds_bytes, stream, p);
+  else if (mode == kModeAdjoint)
+    hipLaunchKernelGGL((trace_frame_kernel<256, 4, false, true, kModeAdjoint>), dim3(shape.grid), dim3(256),
+                       shape.lds_bytes, stream, p);
+  else
+    return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 uint32_t compaction_chunks(uint32_t n) { return (n + kCompactChunk - 1u) / kCompactChunk; }

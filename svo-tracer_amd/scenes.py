@@ -8,6 +8,8 @@
   depth 8, world [-1.25, 1.25]^3 mapped onto the octree's unit cube;
 * ``config2``: 1920x1080, depth-12 fBm heightfield (6 octaves, lacunarity 2,
   gain 0.5, seed 7) over a 4096^2 grid, 100 frames along a seeded orbit;
+* ``config4``: 64 cameras around the config-2 scene, resolutions log-uniform between
+  256x256 and 1920x1080, poses on a hemisphere, apertures 1-4 px (seed 11);
 * ``config5``: depth-10 sphere+noise scene (seed 3), 32 views of 512x512.
 
 All randomness is SplitMix64 with the stated seeds.  Camera facing rows are
@@ -178,6 +180,30 @@ def config2_camera(frame: int, width: int = 1920, height: int = 1080, frames: in
     fov_x = math.radians(60.0)
     t = (frame % 65533) + 1 if tick is None else tick
     return trace_input(eye, look_at(eye, target), width, height, fov_x, fov_x * height / width, 4, t)
+
+
+def config4_cameras(count: int = 64, tick: int = 1, seed: int = 11) -> Tuple[List[TraceInput], List[float]]:
+    """Batched cameras around the config-2 heightfield: resolution (256*7.5^t, 256*4.21875^t)
+    with t uniform (pixel count log-uniform between 256^2 and 1920x1080), eye on a hemisphere
+    of radius 0.3-0.4 above the scene centre, aperture an integer 1..4 pixels."""
+    rng = SplitMix64(seed)
+    cams, apertures = [], []
+    centre = (0.5, 0.5, 0.40)
+    for _ in range(count):
+        t = rng.uniform()
+        w = int(round(256.0 * (1920.0 / 256.0) ** t))
+        h = int(round(256.0 * (1080.0 / 256.0) ** t))
+        z = 0.35 + 0.6 * rng.uniform()
+        phi = 2.0 * math.pi * rng.uniform()
+        radius = 0.30 + 0.10 * rng.uniform()
+        rr = math.sqrt(max(0.0, 1.0 - z * z))
+        eye = (centre[0] + radius * rr * math.cos(phi), centre[1] + radius * rr * math.sin(phi),
+               centre[2] + radius * z)
+        target = (centre[0] + 0.1 * (rng.uniform() - 0.5), centre[1] + 0.1 * (rng.uniform() - 0.5), centre[2])
+        fov_x = math.radians(60.0)
+        cams.append(trace_input(eye, look_at(eye, target), w, h, fov_x, fov_x * h / w, 4, tick))
+        apertures.append(float(1 + int(rng.uniform() * 4.0) % 4))
+    return cams, apertures
 
 
 def config5_scene() -> SceneSpec:
