@@ -113,8 +113,14 @@ def batched_leg(svo, cm, steps: int, warmup: int, usages0, opts) -> dict:
 
     ms_batch = timed(batch)
     ms_loop = timed(loop)
+    # launch options for the batched frame chosen by the autotuner on the frame itself
+    cm.WriteBuffer(B.Usages, usages0)
+    tuned, tuned_ms = cm.Autotune(frame_cams(0), apertures, frames=3, **graph_opts(opts))
+    tuned = {k: v for k, v in tuned.items() if v}
+    log(f"autotuned batched options: {tuned} ({tuned_ms:.2f} ms/frame)")
+    ms_batch_tuned = timed(lambda f: cm.TraceBatch(frame_cams(f), apertures, **tuned))
     # the same two submissions captured once into HIP graphs and replayed per step
-    g_batch = cm.CaptureFrame(frame_cams(0), apertures, per_camera=False, **graph_opts(opts))
+    g_batch = cm.CaptureFrame(frame_cams(0), apertures, per_camera=False, **tuned)
     g_loop = cm.CaptureFrame(frame_cams(0), apertures, per_camera=True, **graph_opts(opts))
     try:
         ms_batch_graph = timed(lambda f: g_batch.Launch(frame_cams(f)))
@@ -129,8 +135,10 @@ def batched_leg(svo, cm, steps: int, warmup: int, usages0, opts) -> dict:
         "ms_per_step": round(ms_batch_graph, 3),
         "cameras": len(cams),
         "cones_per_step": n,
-        "submission": "hipGraph replay of one batched launch",
+        "submission": "hipGraph replay of one batched launch with autotuned options",
+        "autotuned_options": tuned,
         "direct_launch": {"value": rate(ms_batch), "ms_per_step": round(ms_batch, 3)},
+        "direct_launch_autotuned": {"value": rate(ms_batch_tuned), "ms_per_step": round(ms_batch_tuned, 3)},
         "per_camera_loop": {"value": rate(ms_loop), "ms_per_step": round(ms_loop, 3)},
         "per_camera_loop_graph": {"value": rate(ms_loop_graph), "ms_per_step": round(ms_loop_graph, 3)},
         "workload": "64 cameras, resolutions log-uniform 256x256..1920x1080, apertures 1-4 px, seed 11, "

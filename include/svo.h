@@ -296,8 +296,8 @@ enum {
   SVO_FRAME_STATE = 1u << 0,  /* also write final Locations (ulong3, 32-B stride) and Depths */
   SVO_FRAME_QUEUES = 1u << 1, /* also write BackgroundQueue/MaterialQueue (+ their ID counters) */
   SVO_FRAME_NO_XCD_SPLIT = 1u << 2,
-  SVO_FRAME_CAMERA_TABLE = 1u << 3 /* read camera parameters from a device table (the batched kernel) also
-                                      for a single camera */
+  SVO_FRAME_CAMERA_TABLE = 1u << 3 /* accepted for compatibility: camera parameters always come from a
+                                      device table (one record per camera) */
 };
 
 /* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block
@@ -327,6 +327,14 @@ int svo_frame_graph_create(svo_context *ctx, const svo_trace_input *inputs, cons
                            svo_frame_graph **out);
 int svo_frame_graph_launch(svo_context *ctx, svo_frame_graph *graph, const svo_trace_input *inputs);
 int svo_frame_graph_destroy(svo_context *ctx, svo_frame_graph *graph);
+
+/* Picks svo_frame_options for a (batched) frame by timing `frames` real frames per
+ * candidate (coordinate descent over workgroup_size 64..512, work_chunk 64..1024 cones per
+ * claim and refill_lanes 16..64), starting from *base (may be NULL).  All candidates give
+ * the same results, and repeating a frame leaves the same state, so the caller's frame can
+ * be tuned in place.  No reference counterpart. */
+int svo_autotune(svo_context *ctx, const svo_trace_input *inputs, const float *apertures, uint32_t n_cameras,
+                 uint32_t frames, const svo_frame_options *base, svo_frame_options *best, float *best_ms);
 
 /* ---- differentiable compositing (no reference counterpart) ----
  * Every material hit of a cone (the block the reference would shade) becomes a layer with

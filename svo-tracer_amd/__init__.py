@@ -349,6 +349,8 @@ _PROTOS = {
     "svo_composite": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(CompositeOptions),
                       C.POINTER(C.c_double)],
     "svo_adam_step": [C.c_void_p, C.POINTER(AdamOptions)],
+    "svo_autotune": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(FrameOptions),
+                     C.POINTER(FrameOptions), C.POINTER(C.c_float)],
     "svo_timer_begin": [C.c_void_p],
     "svo_timer_end": [C.c_void_p, C.POINTER(C.c_float)],
     "svo_kernel_timing_enable": [C.c_void_p, C.c_int],
@@ -643,6 +645,20 @@ class ComputeManager:
         _check(lib().svo_composite(self._h, arr, None if ap is None else ap.ctypes.data, n, C.byref(o),
                                    C.byref(loss)), "Composite")
         return loss.value
+
+    def Autotune(self, trace_inputs, apertures=None, frames: int = 3, **base) -> tuple:
+        """svo_autotune: (best launch options as a dict, ms per frame with them)."""
+        n = len(trace_inputs)
+        arr = (TraceInput * n)(*trace_inputs)
+        ap = None if apertures is None else np.ascontiguousarray(apertures, dtype=np.float32)
+        b, best = FrameOptions(), FrameOptions()
+        for k, v in base.items():
+            setattr(b, k, v)
+        ms = C.c_float(0.0)
+        _check(lib().svo_autotune(self._h, arr, None if ap is None else ap.ctypes.data, n, frames, C.byref(b),
+                                  C.byref(best), C.byref(ms)), "Autotune")
+        keys = ("workgroup_size", "work_chunk", "refill_lanes", "stack_levels", "waves_per_simd", "flags")
+        return {k: getattr(best, k) for k in keys}, ms.value
 
     def AdamStep(self, step: int, learning_rate: float = 0.01, beta1: float = 0.9, beta2: float = 0.999,
                  epsilon: float = 1e-8) -> None:

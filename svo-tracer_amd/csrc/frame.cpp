@@ -160,7 +160,7 @@ int validate_batch(const svo_trace_input *ins, const float *apertures, uint32_t 
 }
 
 // Everything of a frame but its submission: buffers checked, scratch sized, camera tables
-// uploaded, launch shape chosen.  Camera records (multi) are written to `host_cams` and
+// uploaded, launch shape chosen.  Camera records are written to `host_cams` and
 // read by the kernel from `dev_cams`.
 int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
                const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
@@ -171,7 +171,6 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   const svo_trace_input &in = ins[0];
   const uint32_t B = in.base_depth;
-  const bool multi = n_cams > 1 || (o.flags & SVO_FRAME_CAMERA_TABLE) != 0;
   if (!fb.bases.ptr) return fail(SVO_ERR_UNKNOWN_BUFFER, "trace frame: buffer Bases not initialised");
   if (!fb.blocks.ptr) return fail(SVO_ERR_UNKNOWN_BUFFER, "trace frame: buffer Blocks not initialised");
   const uint32_t n_bases = pow_sum(B);
@@ -232,22 +231,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   p.totals = (unsigned long long *)(small + kTotalsOffset);
 
   uint32_t tiles = 0;
-  if (!multi) {
-    if ((rc = update_camera_table(ctx, in))) return rc;
-    p.lut = (const float *)ctx->scratch_lut->ptr;
-    CamRec one;
-    std::memset(&one, 0, sizeof one);
-    fill_camera_poses(ins, apertures, 1, &one);
-    for (int k = 0; k < 9; k++) p.facing[k] = one.facing[k];
-    for (int k = 0; k < 3; k++) p.origin[k] = one.origin[k];
-    p.dof[0] = one.dof[0];
-    p.dof[1] = one.dof[1];
-    p.cone_fov = one.cone_fov;
-    p.W = in.screen_size[0];
-    p.H = in.screen_size[1];
-    p.tiles_x = (p.W + 7) / 8;
-    tiles = p.tiles_x * ((p.H + 7) / 8);
-  } else {
+  {
     std::vector<uint32_t> offsets;
     if (lut_dev) {
       offsets.assign(lut_offsets, lut_offsets + n_cams);
@@ -274,7 +258,6 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
     p.n_cams = n_cams;
   }
   p.B = B;
-  p.tick = in.tick;
   // eight bands of tiles, one per XCD (work-group b runs on XCD b mod 8); without the split
   // every wave claims from one shared range
   const bool split = (o.flags & SVO_FRAME_NO_XCD_SPLIT) == 0;
@@ -286,7 +269,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
 
   bool bases_in_lds = false;
   uint32_t levels = 1;
-  if ((rc = choose_launch(ctx, B, n_bases, multi ? (size_t)n_cams * sizeof(CamRec) : 0, o, &bases_in_lds, &levels,
+  if ((rc = choose_launch(ctx, B, n_bases, (size_t)n_cams * sizeof(CamRec), o, &bases_in_lds, &levels,
                           &plan->shape)))
     return rc;
   p.bases_in_lds = bases_in_lds ? 1u : 0u;
@@ -313,11 +296,11 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
     c.totals = p.totals;
   }
   plan->cones = N;
-  plan->n_cams = multi ? n_cams : 0;
+  plan->n_cams = n_cams;
   return SVO_OK;
 }
 
-// The stream work of one planned frame: camera records (multi), counter reset, trace
+// The stream work of one planned frame: camera records, counter reset, trace
 // kernel, compaction.  No host synchronisation and no memset nodes (capturable).
 int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool timing) {
   if (plan.n_cams) {

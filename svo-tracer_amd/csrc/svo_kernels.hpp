@@ -36,14 +36,9 @@ struct FrameParams {
   uint16_t *usage16;  // svo_usage array viewed as u16: tick of group g at [4g]
   uint32_t n_usages;
   // camera
-  const float *lut;  // cos_u[W], sin_u[W], cos_v[H], sin_v[H]
-  float facing[9];
-  float origin[3];
-  float dof[2];
-  float cone_fov;
-  uint32_t W, H, B, tick;
+  const float *lut;  // per camera: cos_u[W], sin_u[W], cos_v[H], sin_v[H] at its lut_offset
+  uint32_t B;
   // work distribution
-  uint32_t tiles_x;
   uint32_t range_begin[9];
   uint32_t *counters;
   unsigned long long *totals;  // [0] base steps [1] block steps [2] child requests
@@ -56,8 +51,8 @@ struct FrameParams {
   uint8_t *depth_out;
   uint64_t *loc_out;  // ulong3 with a 32-byte stride
   svo_child_request *req_sparse;
-  // batched cameras (n_cams > 0: camera parameters come from this table, tiles of all
-  // cameras form one work space, outputs are indexed by the concatenated cone index)
+  // cameras (one or a batch): the tiles of all cameras form one work space, outputs are
+  // indexed by the concatenated cone index
   const CamRec *cams;
   uint32_t n_cams;
   // differentiable compositing (composite modes only)

@@ -37,24 +37,25 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 // WAVES > 0 asks the compiler for that many waves per SIMD (a register budget).
 // HASH: the visit-sequence hash is computed only when its output is bound.
-// MULTI: cameras come from p.cams (staged in LDS); otherwise from the kernel argument.
+// Camera parameters come from the table p.cams (one camera or a batch), staged in LDS:
+// kernel arguments stay small, so the scalar registers are not spilled.
 // MODE: kModePlain ends a cone at its first material hit (the reference's frame);
 // kModeComposite makes every hit a layer composited front to back and continues past the
 // cell (up to p.max_layers layers); kModeAdjoint also back-propagates the L2 image loss to
 // the layers' colour and opacity.
-template <int WG, int WAVES, bool HASH, bool MULTI, int MODE = kModePlain>
+template <int WG, int WAVES, bool HASH, int MODE = kModePlain>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
 trace_frame_kernel(const FrameParams p) {
   extern __shared__ uint32_t smem[];
   uint16_t *s_bases = reinterpret_cast<uint16_t *>(smem);
   const uint32_t bases_words = p.bases_in_lds ? (p.n_bases + 1u) / 2u : 0u;
   uint32_t *s_cam = smem + bases_words;
-  const uint32_t cam_words = MULTI ? p.n_cams * kCamWords : 0u;
+  const uint32_t cam_words = p.n_cams * kCamWords;
   uint32_t *s_stack = s_cam + cam_words;
   if (p.bases_in_lds) {
     for (uint32_t i = threadIdx.x; i < p.n_bases; i += WG) s_bases[i] = p.bases[i];
   }
-  if (MULTI) {
+  {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(p.cams);
     for (uint32_t i = threadIdx.x; i < cam_words; i += WG) s_cam[i] = src[i];
   }
@@ -63,13 +64,13 @@ trace_frame_kernel(const FrameParams p) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t B = p.B, S = p.stack_levels;
   const uint32_t xcd = blockIdx.x & 7u;
-  // camera parameters: word w of camera c (MULTI) or of the single camera
+  // camera parameters: word w of camera c
   auto camf = [&](uint32_t c, uint32_t w) -> float { return __uint_as_float(s_cam[c * kCamWords + w]); };
   auto camu = [&](uint32_t c, uint32_t w) -> uint32_t { return s_cam[c * kCamWords + w]; };
-  uint32_t cam = 0;  // the lane's camera (MULTI)
-  // the frame tick: with a camera table it travels with the records (word 21 of camera 0),
-  // so a captured launch can be replayed for later frames
-  const uint32_t tick = MULTI ? camu(0, 21) : p.tick;
+  uint32_t cam = 0;  // the lane's camera
+  // the frame tick travels with the camera records (word 21 of camera 0), so a captured
+  // launch can be replayed for later frames
+  const uint32_t tick = camu(0, 21);
 
   uint32_t range_step = 0;  // wave-uniform: ranges tried, in order xcd, xcd+1, ...
   bool fetching = true;     // wave-uniform
@@ -277,29 +278,21 @@ trace_frame_kernel(const FrameParams p) {
         if (!active && k < cur_end) {
           uint32_t tile = cur_t0 + (k >> 6);
           const uint32_t l = k & 63u;
-          uint32_t W = p.W, H = p.H, tiles_x = p.tiles_x, ray0 = 0, lut0 = 0;
-          float f[9];
-          if (MULTI) {
-            // the camera whose tiles hold this one: the last with tile_begin <= tile
-            uint32_t lo = 0, hi = p.n_cams;
-            while (hi - lo > 1u) {
-              const uint32_t mid = (lo + hi) >> 1;
-              if (camu(mid, 18) <= tile)
-                lo = mid;
-              else
-                hi = mid;
-            }
-            cam = lo;
-            tile -= camu(cam, 18);
-            W = camu(cam, 15);
-            H = camu(cam, 16);
-            tiles_x = camu(cam, 17);
-            ray0 = camu(cam, 19);
-            lut0 = camu(cam, 20);
-            for (int j = 0; j < 9; j++) f[j] = camf(cam, j);
-          } else {
-            for (int j = 0; j < 9; j++) f[j] = p.facing[j];
+          // the camera whose tiles hold this one: the last with tile_begin <= tile
+          uint32_t lo = 0, hi = p.n_cams;
+          while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (camu(mid, 18) <= tile)
+              lo = mid;
+            else
+              hi = mid;
           }
+          cam = lo;
+          tile -= camu(cam, 18);
+          const uint32_t W = camu(cam, 15), H = camu(cam, 16), tiles_x = camu(cam, 17), ray0 = camu(cam, 19),
+                         lut0 = camu(cam, 20);
+          float f[9];
+          for (int j = 0; j < 9; j++) f[j] = camf(cam, j);
           const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
           const uint32_t x = tx * 8u + (l & 7u), y = ty * 8u + (l >> 3);
           if (x < W && y < H) {
@@ -325,10 +318,9 @@ trace_frame_kernel(const FrameParams p) {
               cR = cG = cB = 0.0f;
               nlay = 0;
             }
-            const float o[3] = {MULTI ? camf(cam, 9) : p.origin[0], MULTI ? camf(cam, 10) : p.origin[1],
-                                MULTI ? camf(cam, 11) : p.origin[2]};
+            const float o[3] = {camf(cam, 9), camf(cam, 10), camf(cam, 11)};
             const float d[3] = {dx, dy, dz}, inv[3] = {ix, iy, iz};
-            if (trace_into_volume(o, d, inv, MULTI ? camf(cam, 12) : p.cone_fov, lx, ly, lz)) {
+            if (trace_into_volume(o, d, inv, camf(cam, 12), lx, ly, lz)) {
               depth = 1;
               active = true;
             } else {
@@ -399,10 +391,8 @@ trace_frame_kernel(const FrameParams p) {
         }
       } else {
         // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
-        const float cone = MULTI ? cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), lx, ly, lz, camf(cam, 12),
-                                           camf(cam, 13), camf(cam, 14))
-                                 : cone_at(p.origin[0], p.origin[1], p.origin[2], lx, ly, lz, p.cone_fov, p.dof[0],
-                                           p.dof[1]);
+        const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), lx, ly, lz, camf(cam, 12),
+                                   camf(cam, 13), camf(cam, 14));
         bool hit = false, child = false, tree_size = false;
         if (cone > exp2_neg(B + 2u)) {
           depth = B + 2u;
@@ -622,32 +612,21 @@ __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
 
 template <int WG, int WAVES>
 int launch_trace_wg(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
-  if (p.n_cams && p.hashes)
-    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
-                       stream, p);
-  else if (p.n_cams)
-    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
-                       stream, p);
-  else if (p.hashes)
-    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
-                       stream, p);
+  if (p.hashes)
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, true>), dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
   else
-    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes,
-                       stream, p);
+    hipLaunchKernelGGL((trace_frame_kernel<WG, WAVES, false>), dim3(shape.grid), dim3(WG), shape.lds_bytes, stream, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// Occupancy of the variant with the most registers bounds the grid of every variant.
+// Occupancy of the hashing variant (never fewer registers than the plain one) bounds the
+// grid of both.
 template <int WG, int WAVES>
 int occupancy_wg(size_t lds, int *blocks_per_cu) {
-  int a = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, trace_frame_kernel<WG, WAVES, true, false>, WG, lds) !=
-          hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_frame_kernel<WG, WAVES, true, true>, WG, lds) !=
-          hipSuccess)
-    return -1;
-  *blocks_per_cu = a < b ? a : b;
-  return 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_frame_kernel<WG, WAVES, true>, WG, lds) ==
+                 hipSuccess
+             ? 0
+             : -1;
 }
 
 template <int WAVES>
@@ -697,9 +676,9 @@ int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream
 // they are compiled for 4 waves per SIMD.
 int composite_occupancy(size_t lds_bytes, int *blocks_per_cu) {
   int a = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, trace_frame_kernel<256, 4, false, true, kModeComposite>, 256,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, trace_frame_kernel<256, 4, false, kModeComposite>, 256,
                                                    lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_frame_kernel<256, 4, false, true, kModeAdjoint>, 256,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_frame_kernel<256, 4, false, kModeAdjoint>, 256,
                                                    lds_bytes) != hipSuccess)
     return -1;
   *blocks_per_cu = a < b ? a : b;
@@ -709,10 +688,10 @@ int composite_occupancy(size_t lds_bytes, int *blocks_per_cu) {
 int launch_composite(const FrameParams &p, const LaunchShape &shape, int mode, hipStream_t stream) {
   if (shape.workgroup != 256 || !p.n_cams) return -1;
   if (mode == kModeComposite)
-    hipLaunchKernelGGL((trace_frame_kernel<256, 4, false, true, kModeComposite>), dim3(shape.grid), dim3(256),
+    hipLaunchKernelGGL((trace_frame_kernel<256, 4, false, kModeComposite>), dim3(shape.grid), dim3(256),
                        shape.lds_bytes, stream, p);
   else if (mode == kModeAdjoint)
-    hipLaunchKernelGGL((trace_frame_kernel<256, 4, false, true, kModeAdjoint>), dim3(shape.grid), dim3(256),
+    hipLaunchKernelGGL((trace_frame_kernel<256, 4, false, kModeAdjoint>), dim3(shape.grid), dim3(256),
                        shape.lds_bytes, stream, p);
   else
     return -1;

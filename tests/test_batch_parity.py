@@ -195,3 +195,20 @@ def test_captured_frame_replays_match_direct_frames(svo, gpu, scene, per_camera)
             assert first_mismatch(got_usages, want_usages) is None
     finally:
         graph.Dispose()
+
+
+def test_autotune_keeps_results(svo, orc, gpu, scene):
+    """The tuner's frames leave the state one frame leaves, and the tuned options give the
+    same outputs as the defaults."""
+    tree, bases, blocks, usages = scene
+    cams = _cameras(svo, [(64, 48), (40, 24)])
+    apertures = [1.0, 2.0]
+    want = gpu_batch(svo, gpu, bases, blocks, usages.copy(), cams, apertures)
+    best, ms = gpu.Autotune(cams, apertures, frames=2, flags=svo.FRAME_QUEUES)
+    assert ms > 0 and best["workgroup_size"] in (0, 64, 128, 256, 512)
+    got_usages = gpu.ReadBuffer(svo.BufferName.Usages, svo.USAGE_DTYPE, usages.size)
+    assert first_mismatch(got_usages, want["usages"]) is None
+    opts = {k: v for k, v in best.items() if v and k != "flags"}
+    got = gpu_batch(svo, gpu, bases, blocks, usages.copy(), cams, apertures, **opts)
+    for key in ("hit", "visits", "status", "rgba", "hash", "requests", "background_queue", "material_queue"):
+        assert first_mismatch(got[key], want[key]) is None, key
