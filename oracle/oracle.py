@@ -108,6 +108,10 @@ def lib() -> C.CDLL:
         L.orc_graft.argtypes = [C.POINTER(Pool), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                 C.c_uint32, C.POINTER(TreeStats)]
         L.orc_graft.restype = C.c_int
+        L.orc_reclaim.argtypes = [C.POINTER(Pool), C.c_void_p]
+        L.orc_reclaim.restype = C.c_uint32
+        L.orc_compact.argtypes = [C.POINTER(Pool), C.c_void_p]
+        L.orc_compact.restype = C.c_uint32
         L.orc_tick_age.argtypes = [C.c_uint16, C.c_uint16]
         L.orc_tick_age.restype = C.c_uint32
         _lib = L
@@ -351,6 +355,18 @@ class TreePool:
         if rc != 0:
             raise RuntimeError(f"orc_graft failed: {rc}")
         return {k: getattr(st, k) for k, _ in TreeStats._fields_}
+
+
+def _pool_method_reclaim(self, inp: np.ndarray) -> int:
+    return int(lib().orc_reclaim(C.byref(self._pool()), inp.ctypes.data))
+
+
+def _pool_method_compact(self, inp: np.ndarray) -> int:
+    return int(lib().orc_compact(C.byref(self._pool()), inp.ctypes.data))
+
+
+TreePool.reclaim = _pool_method_reclaim
+TreePool.compact = _pool_method_compact
 
 
 def tick_age(tick: int, now: int) -> int:

@@ -535,3 +535,113 @@ int orc_graft(orc_pool *p, const orc_update_input *in, const orc_grafting *rec, 
   if (stats) *stats = st;
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* garbage collection: reclaim stale groups, compact the pool                 */
+/* ------------------------------------------------------------------------ */
+
+/* Frees every stale group (age > offset, as collection defines it): cut from its parent,
+ * its subtree freed, its own tick 0.  Returns the number of groups reclaimed. */
+uint32_t orc_reclaim(orc_pool *p, const orc_update_input *in) {
+  uint8_t *claimed = (uint8_t *)calloc(p->n_usages ? p->n_usages : 1, 1);
+  uint32_t *list = (uint32_t *)malloc(sizeof(uint32_t) * (p->n_usages ? p->n_usages : 1));
+  const uint16_t now = in->tick;
+  const uint32_t ng = pool_groups(p, in);
+  uint32_t k = 0;
+  for (uint32_t g = 0; g < ng; g++) {
+    const uint16_t t = p->usages[g].tick;
+    if (t != 0 && t < TICK_MULTI && orc_tick_age(t, now) > in->offset) list[k++] = g;
+  }
+  links_t roots = {0, 0, 0};
+  for (uint32_t i = 0; i < k; i++) {
+    const uint32_t g = list[i];
+    claimed[g] = 1;
+    const uint32_t pb = p->usages[g].parent;
+    if (pb < p->n_blocks && p->blocks[pb].child == 8u * g) p->blocks[pb].child = U32MAX;
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t b = 8u * g + j;
+      if (b >= p->n_blocks) break;
+      const uint32_t c = p->blocks[b].child;
+      if (c != U32MAX) links_push(&roots, c >> 3, b);
+    }
+  }
+  free_subtrees(p, &roots, claimed, now);
+  free(roots.v);
+  for (uint32_t i = 0; i < k; i++) {
+    p->usages[list[i]].tick = 0;
+    p->usages[list[i]].parent = U32MAX;
+  }
+  free(list);
+  free(claimed);
+  return k;
+}
+
+/* Compaction: with L live groups (tick != 0), the live single-parent groups at or above L
+ * move, in ascending order, into the free groups below L, in ascending order; blocks move
+ * with their groups, Child fields and Usage.Parent back-pointers follow.  Returns the number
+ * of groups moved. */
+uint32_t orc_compact(orc_pool *p, const orc_update_input *in) {
+  (void)in;
+  const uint32_t n = p->n_usages;
+  uint32_t live = 0;
+  for (uint32_t g = 0; g < n; g++) live += p->usages[g].tick != 0;
+  uint32_t *remap = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *from = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *to = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  for (uint32_t g = 0; g < n; g++) remap[g] = g;
+  uint32_t nf = 0, nt = 0;
+  for (uint32_t g = live; g < n; g++) {
+    const uint16_t t = p->usages[g].tick;
+    if (t != 0 && t < TICK_MULTI) from[nf++] = g;
+  }
+  for (uint32_t g = 0; g < live && g < n; g++)
+    if (p->usages[g].tick == 0) to[nt++] = g;
+  const uint32_t m = nf < nt ? nf : nt;
+  for (uint32_t i = 0; i < m; i++) remap[from[i]] = to[i];
+#define REMAP_BLOCK(b) (remap[(b) >> 3] * 8u + ((b)&7u))
+  for (uint32_t i = 0; i < m; i++) {
+    const uint32_t o = from[i], nw = to[i];
+    for (uint32_t k = 0; k < 8; k++) {
+      orc_block b = p->blocks[8u * o + k];
+      if (b.child != U32MAX && (b.child >> 3) < n) b.child = REMAP_BLOCK(b.child);
+      p->blocks[8u * nw + k] = b;
+    }
+    orc_usage u = p->usages[o];
+    if (u.parent < p->n_blocks) u.parent = REMAP_BLOCK(u.parent);
+    p->usages[nw] = u;
+  }
+  /* links into moved groups from groups that stay: the parent block's Child, the
+   * children's Usage.Parent (or their parent-list entries when they are multi-parent) */
+  uint8_t *target = (uint8_t *)calloc(n ? n : 1, 1);
+  for (uint32_t i = 0; i < m; i++) target[to[i]] = 1;
+  for (uint32_t i = 0; i < m; i++) {
+    const uint32_t o = from[i], nw = to[i];
+    const uint32_t pb = p->usages[nw].parent;
+    if (pb < p->n_blocks && remap[pb >> 3] == (pb >> 3)) p->blocks[pb].child = 8u * nw;
+    for (uint32_t k = 0; k < 8; k++) {
+      const uint32_t c = p->blocks[8u * nw + k].child;
+      if (c == U32MAX || (c >> 3) >= n) continue;
+      const uint32_t cg = c >> 3;
+      if (target[cg]) continue;
+      if (p->usages[cg].tick < TICK_MULTI) {
+        if (p->usages[cg].parent == 8u * o + k) p->usages[cg].parent = 8u * nw + k;
+      } else if (p->usages[cg].tick == TICK_MULTI && p->parents) {
+        uint32_t e = p->usages[cg].parent, steps = 0;
+        while (e < p->n_parents && steps++ < p->n_parents) {
+          if (p->parents[e].parent_address == 8u * o + k) p->parents[e].parent_address = 8u * nw + k;
+          e = p->parents[e].next_element;
+        }
+      }
+    }
+  }
+  free(target);
+  for (uint32_t i = 0; i < m; i++) {
+    p->usages[from[i]].tick = 0;
+    p->usages[from[i]].parent = U32MAX;
+  }
+#undef REMAP_BLOCK
+  free(remap);
+  free(from);
+  free(to);
+  return m;
+}

@@ -113,6 +113,14 @@ int orc_graft(orc_pool *pool, const orc_update_input *in, const orc_grafting *re
               const orc_block *graft_blocks, uint32_t n_graft_blocks, const uint64_t *locations,
               uint32_t n_locations, orc_tree_stats *stats);
 
+/* Garbage collection: frees every stale group (tick age > offset; cut from its parent, its
+ * subtree freed); returns the number reclaimed. */
+uint32_t orc_reclaim(orc_pool *pool, const orc_update_input *in);
+/* Compaction: with L live groups, live single-parent groups at or above L move in ascending
+ * order into the free groups below L in ascending order; links follow.  Returns the number
+ * moved. */
+uint32_t orc_compact(orc_pool *pool, const orc_update_input *in);
+
 /* Age of a tick at frame `now` on the 1..65533 tick cycle (StateManager.cs:113-122). */
 uint32_t orc_tick_age(uint16_t tick, uint16_t now);
 

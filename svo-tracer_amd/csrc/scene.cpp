@@ -16,6 +16,7 @@
 #include <thread>
 
 #include "svo_internal.hpp"
+#include "svo_terrain.hpp"
 
 namespace svo {
 namespace {
@@ -295,6 +296,26 @@ class SphereNoise : public Geometry {
   }
 };
 
+// Terrain parameters of an SVO_GEOMETRY_FBM_HEIGHTFIELD description (defaults applied).
+terrain::Spec terrain_spec(const svo_geometry &g) {
+  terrain::Spec s;
+  uint32_t res = g.resolution ? g.resolution : 4096, lres = 0;
+  while ((1u << lres) < res) lres++;
+  s.seed = g.seed;
+  s.octaves = g.octaves ? g.octaves : 6;
+  s.lres = lres;
+  s.lacunarity = g.lacunarity > 0 ? g.lacunarity : 2.0;
+  s.gain = g.gain > 0 ? g.gain : 0.5;
+  s.frequency = g.frequency > 0 ? g.frequency : 4.0;
+  s.z_min = g.z_min;
+  s.z_max = g.z_max;
+  if (!(s.z_max > s.z_min)) {
+    s.z_min = 0.25;
+    s.z_max = 0.55;
+  }
+  return s;
+}
+
 // fBm heightfield on an R x R grid (R a power of two) covering the unit square;
 // each grid cell is a column of constant height in [z_min, z_max].  Classification
 // is exact through a min/max pyramid: geometry iff the cell's bottom is below the
@@ -302,20 +323,13 @@ class SphereNoise : public Geometry {
 class Heightfield : public Geometry {
  public:
   uint32_t res = 0, lres = 0;
-  uint64_t seed;
+  terrain::Spec spec;
   std::vector<std::vector<float>> hmin, hmax;  // per pyramid level L (2^L x 2^L)
-  explicit Heightfield(const svo_geometry &g) : seed(g.seed) {
+  explicit Heightfield(const svo_geometry &g) {
     res = g.resolution ? g.resolution : 4096;
     while ((1u << lres) < res) lres++;
     res = 1u << lres;
-    uint32_t oct = g.octaves ? g.octaves : 6;
-    double lac = g.lacunarity > 0 ? g.lacunarity : 2.0, gain = g.gain > 0 ? g.gain : 0.5;
-    double freq = g.frequency > 0 ? g.frequency : 4.0;
-    double zlo = g.z_min, zhi = g.z_max;
-    if (!(zhi > zlo)) {
-      zlo = 0.25;
-      zhi = 0.55;
-    }
+    spec = terrain_spec(g);
     hmin.resize(lres + 1);
     hmax.resize(lres + 1);
     std::vector<float> &top = hmin[lres];
@@ -325,19 +339,7 @@ class Heightfield : public Geometry {
     for (unsigned t = 0; t < nt; t++) {
       pool.emplace_back([&, t]() {
         for (uint32_t j = t; j < res; j += nt)
-          for (uint32_t i = 0; i < res; i++) {
-            double x = (i + 0.5) / res, y = (j + 0.5) / res;
-            double sum = 0, norm = 0, a = 1, f = freq;
-            for (uint32_t o = 0; o < oct; o++) {
-              sum += a * value_noise2(seed + o, x * f, y * f);
-              norm += a;
-              a *= gain;
-              f *= lac;
-            }
-            double v = 0.5 + 0.5 * (sum / norm) * 1.6;
-            v = std::min(1.0, std::max(0.0, v));
-            top[(size_t)j * res + i] = (float)(zlo + (zhi - zlo) * v);
-          }
+          for (uint32_t i = 0; i < res; i++) top[(size_t)j * res + i] = terrain::height(spec, i, j);
       });
     }
     for (auto &th : pool) th.join();
@@ -369,34 +371,12 @@ class Heightfield : public Geometry {
   void classify(const Cell &c, bool *a, bool *g) const override {
     float lo, hi;
     footprint(c, &lo, &hi);
-    double z0 = std::ldexp((double)c.z, -(int)c.level), z1 = std::ldexp((double)c.z + 1.0, -(int)c.level);
-    *g = z0 < (double)hi;
-    *a = z1 > (double)lo;
-  }
-  float sample(int64_t i, int64_t j) const {
-    i = std::max<int64_t>(0, std::min<int64_t>(res - 1, i));
-    j = std::max<int64_t>(0, std::min<int64_t>(res - 1, j));
-    return hmin[lres][(size_t)j * res + i];
+    terrain::classify(c.level, c.z, lo, hi, a, g);
   }
   void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
-    // sample-grid coordinates of the footprint centre and a half-width of >= 1 sample
-    double scale = std::ldexp((double)res, -(int)c.level);
-    double cx = (c.x + 0.5) * scale, cy = (c.y + 0.5) * scale;
-    double hw = std::max(1.0, 0.5 * scale);
-    int64_t x0 = (int64_t)std::floor(cx - hw), x1 = (int64_t)std::floor(cx + hw);
-    int64_t y0 = (int64_t)std::floor(cy - hw), y1 = (int64_t)std::floor(cy + hw);
-    int64_t ci = (int64_t)std::floor(cx), cj = (int64_t)std::floor(cy);
-    double dx = (sample(x1, cj) - sample(x0, cj)) / ((double)(x1 - x0) / res);
-    double dy = (sample(ci, y1) - sample(ci, y0)) / ((double)(y1 - y0) / res);
-    normal_angle(-dx, -dy, 1.0, pitch, yaw);
     float lo, hi;
     footprint(c, &lo, &hi);
-    double t = std::min(1.0, std::max(0.0, ((double)hi - 0.25) / 0.3));
-    uint64_t hs = mix64(seed ^ ((uint64_t)c.level << 58) ^ ((uint64_t)c.x << 38) ^ ((uint64_t)c.y << 19) ^ c.z);
-    int jitter = (int)(hs & 15u);
-    rgb[0] = (uint8_t)std::min(255.0, 60 + 150 * t + jitter);
-    rgb[1] = (uint8_t)std::min(255.0, 110 + 100 * t + jitter);
-    rgb[2] = (uint8_t)std::min(255.0, 50 + 190 * t * t + jitter);
+    terrain::attributes(spec, hmin[lres].data(), c.level, c.x, c.y, c.z, hi, pitch, yaw, rgb);
   }
 };
 
