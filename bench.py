@@ -223,6 +223,75 @@ def adjoint_leg(svo, steps: int, fit_steps: int) -> dict:
     }
 
 
+def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
+    """Streaming: a 3840x2160 camera flies the seeded 300-frame spline over the fBm terrain
+    generated on the device (16384^2 grid) and refined on demand to depth 14 in a fixed pool of
+    2^26 blocks; per frame the trace makes child requests and ticks usages, and the streaming
+    step grafts up to 65,536 bricks, collects groups unused for 30 frames and compacts when
+    fewer than 10 % of the groups are free."""
+    B = svo.BufferName
+    Bd, D, n_groups = 4, 14, 1 << 23
+    n = width * height
+    cm = svo.ComputeManager(0)
+    try:
+        t0 = time.perf_counter()
+        cm.InitDeviceBuffer(B.Bases, sum(8 ** d for d in range(1, Bd + 1)), 2)
+        cm.InitDeviceBuffer(B.Blocks, 8 * n_groups, 16)
+        cm.InitDeviceBuffer(B.Usages, n_groups, 8)
+        cm.InitDeviceBuffer(B.ChildRequests, n, 40)
+        cm.InitBuffer(B.ChildRequestID, np.zeros(1, np.uint32))
+        cm.InitDeviceBuffer(B.OutputImage, n, 16)
+        cm.InitDeviceBuffer(B.HitIndices, n, 4)
+        cm.InitDeviceBuffer(B.VisitCounts, n, 4)
+        terrain = cm.CreateTerrain(svo.scenes.config3_geometry())
+        cm.StreamSeed(terrain, Bd)
+        cm.Wait()
+        setup_s = time.perf_counter() - t0
+        cams = [svo.scenes.config3_camera(f, width, height, frames=frames, base_depth=Bd) for f in range(frames)]
+        trace_ms, update_ms, stats = [], [], []
+        wall0 = time.perf_counter()
+        for f, cam in enumerate(cams):
+            cam.tick = f + 1
+            cam.max_child_request_id = n
+            cm.TimerBegin()
+            cm.TraceFrame(cam)
+            trace_ms.append(cm.TimerEnd())
+            t1 = time.perf_counter()
+            stats.append(cm.StreamUpdate(terrain, Bd, D, f + 1, max_bricks=65536, max_child_request_id=n,
+                                         prune_age=30, gc_free_percent=10))
+            update_ms.append((time.perf_counter() - t1) * 1e3)
+            if f % 50 == 49:
+                log(f"stream frame {f + 1}: trace {np.mean(trace_ms[-50:]):.2f} ms, update "
+                    f"{np.mean(update_ms[-50:]):.2f} ms, free groups {stats[-1]['free_groups']}")
+        cm.Wait()
+        wall_s = time.perf_counter() - wall0
+        terrain.close()
+    finally:
+        cm.Dispose()
+    frame_ms = wall_s * 1e3 / frames
+    return {
+        "value": round(n / (frame_ms / 1e3) / 1e6, 2),
+        "unit": "Mcones/s",
+        "ms_per_step": round(frame_ms, 3),
+        "workload": f"{width}x{height}, {frames}-frame seeded spline (seed 3), fBm terrain 16384^2 generated on "
+                    f"the device, refined on demand to depth {D}, pool 2^26 blocks, <= 65536 bricks/frame, "
+                    "prune age 30 frames, GC below 10 % free",
+        "frames": frames,
+        "trace_ms_mean": round(float(np.mean(trace_ms)), 3),
+        "update_ms_mean": round(float(np.mean(update_ms)), 3),
+        "update_ms_max": round(float(np.max(update_ms)), 3),
+        "bricks_total": int(sum(s["bricks"] for s in stats)),
+        "bricks_per_frame_max": int(max(s["bricks"] for s in stats)),
+        "requests_per_frame_mean": round(float(np.mean([s["requests"] for s in stats])), 1),
+        "gc_runs": int(sum(s["gc_runs"] for s in stats)),
+        "reclaimed_total": int(sum(s["reclaimed"] for s in stats)),
+        "moved_total": int(sum(s["moved"] for s in stats)),
+        "free_groups_last": int(stats[-1]["free_groups"]),
+        "pool_groups": n_groups,
+        "setup_s": round(setup_s, 2),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +308,8 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=0, help="cones a wave claims per atomic")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streaming leg")
+    ap.add_argument("--stream-frames", type=int, default=300)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -362,6 +433,10 @@ def main() -> int:
         log("forward + adjoint leg...")
         line["adjoint"] = adjoint_leg(svo, max(3, min(K, 10)), 200)
         log(f"adjoint: {line['adjoint']['value']} Mcones/s, {line['adjoint']['cost_vs_plain_forward']}x plain forward")
+    if not args.no_stream:
+        log("streaming leg...")
+        line["streaming"] = stream_leg(svo, args.stream_frames)
+        log(f"streaming: {line['streaming']['value']} Mcones/s, update {line['streaming']['update_ms_mean']} ms/frame")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample)...")
         line["cpu_baseline"] = cpu_baseline(svo, tree, cams, args.cpu_seconds)

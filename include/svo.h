@@ -433,6 +433,18 @@ int svo_graft(svo_context *ctx, const svo_update_input_data *input, const svo_gr
 int svo_collect(svo_context *ctx, const svo_update_input_data *input, uint32_t need, uint32_t *groups,
                 uint32_t *count);
 
+/* Garbage collection, step 1 (no reference counterpart; the reference's only eviction is the
+ * stale scan inside graft, wavefront.cl:1277-1305): every stale live group (tick age >
+ * input->offset, tick not 65534/65535) is cut from its parent block, what hangs below it is
+ * freed as prune frees a subtree, and the group gets tick 0.  `reclaimed` (may be NULL)
+ * receives the number of groups reclaimed. */
+int svo_reclaim(svo_context *ctx, const svo_update_input_data *input, uint32_t *reclaimed);
+/* Garbage collection, step 2 (no reference counterpart): with L groups live (tick != 0), the
+ * live single-parent groups at or above L move, in ascending order, into the free groups below
+ * L, in ascending order; blocks move with their group, and Child fields, Usage.Parent
+ * back-pointers and parent-list entries follow.  `moved` (may be NULL) receives the count. */
+int svo_compact(svo_context *ctx, const svo_update_input_data *input, uint32_t *moved);
+
 /* ---- timing on the context's stream (hipEvent based) ---- */
 int svo_timer_begin(svo_context *ctx);
 int svo_timer_end(svo_context *ctx, float *elapsed_ms);
@@ -495,6 +507,68 @@ int svo_context_load_tree(svo_context *ctx, const svo_tree *tree, int with_usage
 /* Separable camera table used for ray generation (replaces native_sin/native_cos of
  * setupInitialTrace, wavefront.cl:563-595): cos_u/sin_u per column, cos_v/sin_v per row. */
 int svo_camera_lut(const svo_trace_input *input, float *cos_u, float *sin_u, float *cos_v, float *sin_v);
+
+/* ---- streaming: procedural terrain generated on the device on demand ---- */
+
+typedef struct svo_terrain svo_terrain;
+
+/* Builds the height grid and its min/max pyramid of an SVO_GEOMETRY_FBM_HEIGHTFIELD
+ * description on the context's device (the same function the host builder samples). */
+int svo_terrain_create(svo_context *ctx, const svo_geometry *geometry, svo_terrain **out);
+int svo_terrain_destroy(svo_terrain *terrain);
+/* Copies the finest height grid (resolution^2 floats, row-major) to the host. */
+int svo_terrain_heights(const svo_terrain *terrain, float *dst, size_t count);
+/* Seed tree in the bound Bases / Blocks / Usages: Bases of levels 1..base_depth and the dense
+ * block level base_depth+2 exactly as svo_tree_build with max_depth = base_depth+2 makes them
+ * (BuildBaseChunks / PopulateInitialBlocks, TreeBuilder.cs:152-227); the rest of Blocks is
+ * zeroed; the dense groups (g < 8^(base_depth+1)) are inviolate and every other group free. */
+int svo_stream_seed(svo_context *ctx, const svo_terrain *terrain, uint32_t base_depth);
+
+enum {
+  SVO_STREAM_NO_GC = 1,   /* never run garbage collection in this update */
+  SVO_STREAM_FORCE_GC = 2 /* run it whatever the free share */
+};
+
+typedef struct {
+  uint32_t base_depth;
+  uint32_t max_depth;            /* deepest block level the terrain is refined to */
+  uint16_t tick;                 /* frame tick (1..65533) */
+  uint16_t pad;
+  uint32_t max_bricks;           /* grafts per update; 0 = 65536 */
+  uint32_t max_child_request_id; /* requests stored by the frame (TraceInput.MaxChildRequestId); 0 = buffer */
+  uint32_t prune_age;            /* a group unused for more frames than this is stale */
+  uint32_t gc_free_percent;      /* collect garbage when free groups < this % of the pool */
+  uint32_t flags;                /* SVO_STREAM_* */
+} svo_stream_options;
+
+typedef struct {
+  uint32_t requests;    /* child requests the frame made (also those beyond capacity) */
+  uint32_t unique;      /* distinct requesting blocks that can take a child level */
+  uint32_t bricks;      /* bricks grafted */
+  uint32_t collected;   /* groups collected for the grafts */
+  uint32_t gc_runs;     /* 1 when garbage collection ran */
+  uint32_t reclaimed;   /* groups reclaimed */
+  uint32_t moved;       /* groups moved by compaction */
+  uint32_t free_groups; /* free groups after the update */
+  uint32_t pool_groups;
+  uint32_t reserved[3];
+} svo_stream_stats;
+
+/* One streaming step after a traced frame (the WorldManager.UpdateGrafting / RunGraft role,
+ * WorldManager.cs:84-103, MainWindow.cs:307-337, fed by requestChild, wavefront.cl:362-375):
+ *  1. the stored child requests (ChildRequests, count in ChildRequestID) are read in order;
+ *     among those asking for a level (TreeSize >= 1: the cone needs detail below the block),
+ *     the first request of every block address whose block is childless, at depth 1 <=
+ *     d < max_depth, and can have children (an octant with both air and geometry,
+ *     CanHaveChildren, TreeBuilder.cs:291-302) is kept;
+ *  2. the first max_bricks of them, in request order, become graft records k (Grafting,
+ *     GraftAddress = the block, Depth 0xFF) of one brick each: the block's 8 children at level
+ *     d+1 from the terrain (GraftingBlocks[8k, 8k+8), MakeBlock rules, TreeBuilder.cs:74-93);
+ *  3. the records are grafted as svo_graft does, collection age threshold prune_age;
+ *  4. when free groups < gc_free_percent % of the pool: svo_reclaim (age > prune_age), then
+ *     svo_compact. */
+int svo_stream_update(svo_context *ctx, const svo_terrain *terrain, const svo_stream_options *options,
+                      svo_stream_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -357,16 +357,88 @@ class TreePool:
         return {k: getattr(st, k) for k, _ in TreeStats._fields_}
 
 
-def _pool_method_reclaim(self, inp: np.ndarray) -> int:
-    return int(lib().orc_reclaim(C.byref(self._pool()), inp.ctypes.data))
+    def reclaim(self, inp: np.ndarray) -> int:
+        """Frees every stale live group (age > offset); returns how many."""
+        return int(lib().orc_reclaim(C.byref(self._pool()), inp.ctypes.data))
+
+    def compact(self, inp: np.ndarray) -> int:
+        """Moves live groups at or above the live count into the holes below it; returns how many."""
+        return int(lib().orc_compact(C.byref(self._pool()), inp.ctypes.data))
 
 
-def _pool_method_compact(self, inp: np.ndarray) -> int:
-    return int(lib().orc_compact(C.byref(self._pool()), inp.ctypes.data))
+# ---------------------------------------------------------------------------
+# streaming: child requests -> bricks -> graft, then garbage collection
+# ---------------------------------------------------------------------------
+
+GRAFTING_DTYPE = np.dtype([("graft_data_address", "<u4"), ("graft_total_size", "<u4"), ("depth", "u1"),
+                           ("pad", "u1", (3,)), ("graft_address", "<u4")])
+assert GRAFTING_DTYPE.itemsize == 16
 
 
-TreePool.reclaim = _pool_method_reclaim
-TreePool.compact = _pool_method_compact
+def stream_requests(requests: np.ndarray, count: int, capacity: int, blocks: np.ndarray, max_depth: int) -> np.ndarray:
+    """Indices of the requests a streaming step turns into bricks, in request order: among the
+    requests asking for a level (TreeSize >= 1; requestChild, wavefront.cl:362-375, also
+    records cones that need nothing below the block, with TreeSize 0), the first request of
+    every block address whose block is in range, childless, at depth 1 <= d < max_depth and
+    able to have children (an octant whose air and geometry bits are both set,
+    CanHaveChildren, TreeBuilder.cs:291-302)."""
+    n = min(int(count), int(capacity), requests.size)
+    seen = set()
+    keep = []
+    child, chunk = blocks["child"], blocks["chunk"]
+    for i in range(n):
+        a = int(requests["address"][i])
+        if int(requests["tree_size"][i]) == 0 or a >= blocks.size or a in seen:
+            continue
+        seen.add(a)
+        d = int(requests["depth"][i])
+        c = int(chunk[a])
+        mixed = any((c >> (2 * k)) & 3 == 3 for k in range(8))
+        if 1 <= d < max_depth and int(child[a]) == U32MAX and mixed:
+            keep.append(i)
+    return np.asarray(keep, np.int64)
+
+
+def stream_update(pool: "TreePool", requests: np.ndarray, count: int, capacity: int, base_depth: int,
+                  max_depth: int, tick: int, bricks_for, max_bricks: int = 65536, prune_age: int = 30,
+                  gc_free_percent: int = 10, flags: int = 0) -> dict:
+    """One streaming step on the oracle pool: the kept requests (stream_requests), at most
+    max_bricks of them, become graft records k = (8k, 8, depth 0xFF, block address) whose
+    bricks `bricks_for(kept_requests)` returns (8 blocks each, the block's children); they
+    are grafted (collection age threshold prune_age); then, when free groups are fewer than
+    gc_free_percent % of the pool (or flags & 2, and never with flags & 1), every stale group
+    is reclaimed and the pool compacted.  The order of these steps is the product's."""
+    keep = stream_requests(requests, count, capacity, pool.blocks, max_depth)
+    nb = min(keep.size, max_bricks)
+    out = {"requests": int(count), "unique": int(keep.size), "bricks": 0, "collected": 0, "gc_runs": 0,
+           "reclaimed": 0, "moved": 0}
+    records = np.zeros(nb, GRAFTING_DTYPE)
+    bricks = np.zeros(0, pool.blocks.dtype)
+    inp = update_input(base_depth, tick, offset=prune_age)
+    inp["graft_size"] = 8 * nb
+    if nb:
+        sel = requests[keep[:nb]]
+        records["graft_data_address"] = 8 * np.arange(nb, dtype=np.uint32)
+        records["graft_total_size"] = 8
+        records["depth"] = 0xFF
+        records["graft_address"] = sel["address"]
+        bricks = np.ascontiguousarray(bricks_for(sel)).astype(pool.blocks.dtype)
+        assert bricks.size == 8 * nb
+        st = pool.graft(inp, records, bricks, np.zeros(0, np.uint64))
+        out["bricks"], out["collected"] = st["placed"], st["collected"]
+    n_groups = pool.usages.size
+    free = int(np.count_nonzero(pool.usages["tick"] == 0))
+    gc = not (flags & 1) and ((flags & 2) or free * 100 < gc_free_percent * n_groups)
+    if gc:
+        out["gc_runs"] = 1
+        out["reclaimed"] = pool.reclaim(inp)
+        out["moved"] = pool.compact(inp)
+        free = int(np.count_nonzero(pool.usages["tick"] == 0))
+    out["free_groups"] = free
+    out["pool_groups"] = n_groups
+    out["records"] = records
+    out["bricks_array"] = bricks
+    return out
 
 
 def tick_age(tick: int, now: int) -> int:

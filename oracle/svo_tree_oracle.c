@@ -576,13 +576,15 @@ uint32_t orc_reclaim(orc_pool *p, const orc_update_input *in) {
   return k;
 }
 
-/* Compaction: with L live groups (tick != 0), the live single-parent groups at or above L
+/* Compaction (over the groups whose blocks lie in the pool): with L live groups (tick != 0),
+ * the live single-parent groups at or above L
  * move, in ascending order, into the free groups below L, in ascending order; blocks move
  * with their groups, Child fields and Usage.Parent back-pointers follow.  Returns the number
  * of groups moved. */
 uint32_t orc_compact(orc_pool *p, const orc_update_input *in) {
   (void)in;
-  const uint32_t n = p->n_usages;
+  /* groups whose 8 blocks all lie in the pool */
+  const uint32_t n = p->n_usages < p->n_blocks / 8u ? p->n_usages : p->n_blocks / 8u;
   uint32_t live = 0;
   for (uint32_t g = 0; g < n; g++) live += p->usages[g].tick != 0;
   uint32_t *remap = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));

@@ -237,6 +237,27 @@ class AdamOptions(C.Structure):
 COMPOSITE_ADJOINT = 1
 COMPOSITE_LAYERS = 2
 
+STREAM_NO_GC = 1
+STREAM_FORCE_GC = 2
+
+
+class StreamOptions(C.Structure):
+    """svo_stream_options: one streaming step after a traced frame."""
+
+    _fields_ = [("base_depth", C.c_uint32), ("max_depth", C.c_uint32), ("tick", C.c_uint16), ("pad", C.c_uint16),
+                ("max_bricks", C.c_uint32), ("max_child_request_id", C.c_uint32), ("prune_age", C.c_uint32),
+                ("gc_free_percent", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class StreamStats(C.Structure):
+    """svo_stream_stats: what one streaming step did."""
+
+    _fields_ = [(k, C.c_uint32) for k in ("requests", "unique", "bricks", "collected", "gc_runs", "reclaimed",
+                                           "moved", "free_groups", "pool_groups")] + [("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
 
 class UpdateInputData(C.Structure):
     """UpdateInputData, wavefront.cl:77-84 (natural 20-byte layout)."""
@@ -370,6 +391,13 @@ _PROTOS = {
     "svo_graft": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                   C.c_void_p],
     "svo_collect": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)],
+    "svo_reclaim": [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)],
+    "svo_compact": [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)],
+    "svo_terrain_create": [C.c_void_p, C.POINTER(Geometry), C.POINTER(C.c_void_p)],
+    "svo_terrain_destroy": [C.c_void_p],
+    "svo_terrain_heights": [C.c_void_p, C.c_void_p, C.c_size_t],
+    "svo_stream_seed": [C.c_void_p, C.c_void_p, C.c_uint32],
+    "svo_stream_update": [C.c_void_p, C.c_void_p, C.POINTER(StreamOptions), C.POINTER(StreamStats)],
 }
 
 
@@ -583,6 +611,43 @@ class ComputeManager:
         _check(lib().svo_collect(self._h, C.byref(ui), need, out.ctypes.data, C.byref(count)), "Collect")
         return out[: count.value]
 
+    def Reclaim(self, update_input) -> int:
+        """svo_reclaim: frees every stale live group (age > UpdateInputData.Offset)."""
+        ui = self._update_input(update_input)
+        n = C.c_uint32(0)
+        _check(lib().svo_reclaim(self._h, C.byref(ui), C.byref(n)), "Reclaim")
+        return n.value
+
+    def Compact(self, update_input) -> int:
+        """svo_compact: moves live groups above the live count into the holes below it."""
+        ui = self._update_input(update_input)
+        n = C.c_uint32(0)
+        _check(lib().svo_compact(self._h, C.byref(ui), C.byref(n)), "Compact")
+        return n.value
+
+    def CreateTerrain(self, geometry: "Geometry") -> "Terrain":
+        """svo_terrain_create: the heightfield's grid and min/max pyramid on this device."""
+        h = C.c_void_p()
+        _check(lib().svo_terrain_create(self._h, C.byref(geometry), C.byref(h)), "CreateTerrain")
+        return Terrain(h, geometry)
+
+    def StreamSeed(self, terrain: "Terrain", base_depth: int) -> None:
+        """svo_stream_seed: Bases, the dense block level and the usage table of a fresh pool."""
+        _check(lib().svo_stream_seed(self._h, terrain.handle, base_depth), "StreamSeed")
+
+    def StreamUpdate(self, terrain: "Terrain", base_depth: int, max_depth: int, tick: int, max_bricks: int = 65536,
+                     max_child_request_id: int = 0, prune_age: int = 30, gc_free_percent: int = 10,
+                     flags: int = 0) -> dict:
+        """svo_stream_update: the frame's child requests become grafted bricks; garbage
+        collection when the pool runs low."""
+        o = StreamOptions()
+        o.base_depth, o.max_depth, o.tick, o.max_bricks = base_depth, max_depth, tick, max_bricks
+        o.max_child_request_id, o.prune_age, o.gc_free_percent, o.flags = (max_child_request_id, prune_age,
+                                                                            gc_free_percent, flags)
+        st = StreamStats()
+        _check(lib().svo_stream_update(self._h, terrain.handle, C.byref(o), C.byref(st)), "StreamUpdate")
+        return st.as_dict()
+
     def LoadTree(self, tree: "Octree", with_usages: bool = True, usage_tick: int = 0) -> None:
         _check(lib().svo_context_load_tree(self._h, tree.handle, 1 if with_usages else 0, usage_tick), "LoadTree")
 
@@ -686,6 +751,32 @@ class ComputeManager:
         n = C.c_uint64(0)
         _check(lib().svo_kernel_timing_get(self._h, C.byref(total), C.byref(n), 1 if reset else 0), "KernelTiming")
         return total.value, n.value
+
+
+class Terrain:
+    """A device terrain (svo_terrain); keep it alive while streaming from it."""
+
+    def __init__(self, handle: C.c_void_p, geometry: "Geometry"):
+        self.handle = handle
+        self.geometry = geometry
+        res = geometry.resolution or 4096
+        self.resolution = 1 << max(0, (res - 1).bit_length())
+
+    def heights(self) -> np.ndarray:
+        out = np.zeros((self.resolution, self.resolution), np.float32)
+        _check(lib().svo_terrain_heights(self.handle, out.ctypes.data, out.size), "TerrainHeights")
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().svo_terrain_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FrameGraph:

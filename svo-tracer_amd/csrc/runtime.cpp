@@ -169,6 +169,7 @@ int svo_destroy(svo_context *ctx) {
   (void)hipEventDestroy(ctx->timer_end);
   for (auto &b : ctx->tree_scratch) b.reset();
   if (ctx->tree_host) (void)hipHostFree(ctx->tree_host);
+  if (ctx->stream_host) (void)hipHostFree(ctx->stream_host);
   if (ctx->cam_copied) (void)hipEventSynchronize(ctx->cam_copied);
   if (ctx->cam_host) (void)hipHostFree(ctx->cam_host);
   if (ctx->cam_copied) (void)hipEventDestroy(ctx->cam_copied);

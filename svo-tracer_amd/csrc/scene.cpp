@@ -296,7 +296,10 @@ class SphereNoise : public Geometry {
   }
 };
 
-// Terrain parameters of an SVO_GEOMETRY_FBM_HEIGHTFIELD description (defaults applied).
+}  // namespace
+
+// Terrain parameters of an SVO_GEOMETRY_FBM_HEIGHTFIELD description (defaults applied); the
+// device terrain of the streaming path uses the same function.
 terrain::Spec terrain_spec(const svo_geometry &g) {
   terrain::Spec s;
   uint32_t res = g.resolution ? g.resolution : 4096, lres = 0;
@@ -315,6 +318,8 @@ terrain::Spec terrain_spec(const svo_geometry &g) {
   }
   return s;
 }
+
+namespace {
 
 // fBm heightfield on an R x R grid (R a power of two) covering the unit square;
 // each grid cell is a column of constant height in [z_min, z_max].  Classification

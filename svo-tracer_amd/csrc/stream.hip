@@ -1,0 +1,450 @@
+// Streaming support on the GPU: the fBm terrain (height grid + min/max pyramid), the
+// seed tree (Bases and the dense block level), child-request consumption into bricks,
+// and the pool's garbage collection (stale-group reclaim is the collection kernels' reclaim
+// mode; compaction is here).
+//
+// Bricks follow TreeBuilder's rules for one level (MakeBlock/MakeChunk,
+// TreeBuilder.cs:74-93, 264-284): the 8 children of a requesting block, each with the
+// chunk of its octants, its colour and normal, and no children of its own.
+#include <algorithm>
+
+#include "svo_device.hpp"
+#include "svo_stream_kernels.hpp"
+#include "svo_terrain.hpp"
+
+namespace svo {
+namespace {
+
+using dev::kU32Max;
+constexpr int kWg = 256;
+
+__device__ __forceinline__ void footprint(const TerrainView &t, uint32_t level, uint32_t x, uint32_t y, float *lo,
+                                          float *hi) {
+  if (level < t.spec.lres) {
+    const size_t k = t.level_offset[level] + (size_t)y * (1ull << level) + x;
+    *lo = t.hmin[k];
+    *hi = t.hmax[k];
+  } else {
+    const uint32_t sh = level - t.spec.lres;
+    const size_t k = (size_t)(y >> sh) * (1ull << t.spec.lres) + (x >> sh);
+    *lo = *hi = t.top[k];
+  }
+}
+
+// MakeChunk, TreeBuilder.cs:264-284: per octant k, bit 2k air, bit 2k+1 geometry.
+__device__ uint16_t make_chunk(const TerrainView &t, uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
+  uint16_t chunk = 0;
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint32_t cx = 2 * x + (k & 1u), cy = 2 * y + ((k >> 1) & 1u), cz = 2 * z + ((k >> 2) & 1u);
+    float lo, hi;
+    footprint(t, level + 1, cx, cy, &lo, &hi);
+    bool a, g;
+    terrain::classify(level + 1, cz, lo, hi, &a, &g);
+    chunk |= (uint16_t)((a ? 1u : 0u) << (2 * k));
+    chunk |= (uint16_t)((g ? 1u : 0u) << (2 * k + 1));
+  }
+  return chunk;
+}
+
+// MakeBlock, TreeBuilder.cs:74-93 (ColourB takes colour[1] and ColourG colour[2], as there).
+__device__ uint4 make_block(const TerrainView &t, uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
+  const uint16_t chunk = make_chunk(t, level, x, y, z);
+  float lo, hi;
+  footprint(t, level, x, y, &lo, &hi);
+  int16_t pitch, yaw;
+  uint8_t rgb[3];
+  terrain::attributes(t.spec, t.top, level, x, y, z, hi, &pitch, &yaw, rgb);
+  uint4 b;
+  b.x = kU32Max;
+  b.y = (uint32_t)chunk | ((uint32_t)(uint16_t)pitch << 16);
+  // yaw @8, R @10, G @11 (= rgb[2]), B @12 (= rgb[1]), opacity @13 = 255, spare, gloss 0
+  b.z = (uint32_t)(uint16_t)yaw | ((uint32_t)rgb[0] << 16) | ((uint32_t)rgb[2] << 24);
+  b.w = (uint32_t)rgb[1] | (255u << 8);
+  return b;
+}
+
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {  // every third bit -> low bits
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030c30c3u;
+  v = (v | (v >> 4)) & 0x0300f00fu;
+  v = (v | (v >> 8)) & 0x030000ffu;
+  v = (v | (v >> 16)) & 0x000003ffu;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// terrain
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kWg) void terrain_top_kernel(terrain::Spec spec, float *top) {
+  const uint32_t res = 1u << spec.lres;
+  const size_t n = (size_t)res * res;
+  for (size_t k = blockIdx.x * (size_t)kWg + threadIdx.x; k < n; k += (size_t)gridDim.x * kWg)
+    top[k] = terrain::height(spec, (uint32_t)(k % res), (uint32_t)(k / res));
+}
+
+__global__ __launch_bounds__(kWg) void terrain_reduce_kernel(const float *in_min, const float *in_max, float *out_min,
+                                                             float *out_max, uint32_t n) {
+  const size_t total = (size_t)n * n;
+  for (size_t k = blockIdx.x * (size_t)kWg + threadIdx.x; k < total; k += (size_t)gridDim.x * kWg) {
+    const size_t j = k / n, i = k % n;
+    const size_t a = (2 * j) * (2 * (size_t)n) + 2 * i, b = a + 1, c = a + 2 * (size_t)n, d = c + 1;
+    out_min[k] = fminf(fminf(in_min[a], in_min[b]), fminf(in_min[c], in_min[d]));
+    out_max[k] = fmaxf(fmaxf(in_max[a], in_max[b]), fmaxf(in_max[c], in_max[d]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// seed tree: Bases (levels 1..B) and the dense level B+2 (BuildBaseChunks /
+// PopulateInitialBlocks, TreeBuilder.cs:152-227)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kWg) void seed_bases_kernel(TerrainView t, uint16_t *bases, uint32_t level,
+                                                         uint32_t offset) {
+  const uint32_t n = 1u << (3 * level);
+  for (uint32_t m = blockIdx.x * kWg + threadIdx.x; m < n; m += gridDim.x * kWg)
+    bases[offset + m] = make_chunk(t, level, compact3(m), compact3(m >> 1), compact3(m >> 2));
+}
+
+__global__ __launch_bounds__(kWg) void seed_dense_kernel(TerrainView t, const uint16_t *bases, uint32_t B,
+                                                         uint4 *blocks) {
+  const uint32_t level = B + 2, n = 1u << (3 * level);
+  const uint32_t base_start = ((1u << (3u * B)) - 1u) / 7u - 1u;  // powSum(B-1)
+  for (uint32_t la = blockIdx.x * kWg + threadIdx.x; la < n; la += gridDim.x * kWg) {
+    const uint32_t bp = la >> 3;  // the level-(B+1) cell
+    uint4 b = make_uint4(0u, 0u, 0u, 0u);
+    if (((bases[base_start + (bp >> 3)] >> ((bp & 7u) * 2)) & 3u) == 3u)
+      b = make_block(t, level, compact3(la), compact3(la >> 1), compact3(la >> 2));
+    blocks[la] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// child requests -> bricks
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t addr_hash(uint32_t a, uint32_t mask) { return (a * 0x9E3779B1u) & mask; }
+
+// First occurrence of every requested address among the requests that ask for a level
+// (the smallest such request index).
+__global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_request *req, uint32_t n, uint32_t *keys,
+                                                            uint32_t *vals, uint32_t mask, uint32_t n_blocks) {
+  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t a = req[i].address;
+  // never keys: addresses out of range, and requests whose cone needs no level below the
+  // block (TreeSize 0, wavefront.cl:362-375); request_flag_kernel rejects them
+  if (a >= n_blocks || req[i].tree_size == 0) return;
+  uint32_t h = addr_hash(a, mask);
+  for (uint32_t probe = 0; probe <= mask; probe++) {
+    uint32_t k = keys[h];
+    if (k == kU32Max) k = atomicCAS(&keys[h], kU32Max, a);
+    if (k == kU32Max || k == a) {
+      atomicMin(&vals[h], i);
+      return;
+    }
+    h = (h + 1u) & mask;
+  }
+}
+
+// 1 when request i is the first for its address and its block can take a child level (the
+// builder refines exactly the blocks below max_depth that can have children).
+__global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_request *req, uint32_t n,
+                                                           const uint32_t *keys, const uint32_t *vals, uint32_t mask,
+                                                           const uint4 *blocks, uint32_t n_blocks,
+                                                           uint32_t max_depth, uint32_t *flags) {
+  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+  if (i >= n) return;
+  const svo_child_request r = req[i];
+  uint32_t h = addr_hash(r.address, mask), first = kU32Max;
+  for (uint32_t probe = 0; probe <= mask; probe++) {
+    if (keys[h] == r.address) {
+      first = vals[h];
+      break;
+    }
+    h = (h + 1u) & mask;
+  }
+  bool ok = first == i && r.address < n_blocks && (uint32_t)r.depth + 1u <= max_depth && r.depth >= 1;
+  if (ok) {
+    const uint4 blk = blocks[r.address];
+    // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
+    const uint32_t chunk = blk.y & 0xffffu;
+    ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
+  }
+  flags[i] = ok ? 1u : 0u;
+}
+
+// Request i (flagged, rank k) becomes graft record k and brick blocks [8k, 8k+8).
+__global__ __launch_bounds__(kWg) void brick_kernel(TerrainView t, const svo_child_request *req, uint32_t n,
+                                                    const uint32_t *flags, const uint32_t *rank, uint32_t limit,
+                                                    svo_grafting *records, uint4 *bricks) {
+  const uint32_t tid = blockIdx.x * kWg + threadIdx.x;
+  const uint32_t i = tid >> 3, c = tid & 7u;
+  if (i >= n || !flags[i]) return;
+  const uint32_t k = rank[i];
+  if (k >= limit) return;
+  const svo_child_request r = req[i];
+  const uint32_t D = r.depth, s = 64u - D;
+  const uint32_t x = (uint32_t)(r.location[0] >> s), y = (uint32_t)(r.location[1] >> s),
+                 z = (uint32_t)(r.location[2] >> s);
+  bricks[8ull * k + c] = make_block(t, D + 1u, 2 * x + (c & 1u), 2 * y + ((c >> 1) & 1u), 2 * z + ((c >> 2) & 1u));
+  if (c == 0) {
+    svo_grafting g;
+    g.graft_data_address = 8u * k;
+    g.graft_total_size = 8u;
+    g.depth = 0xffu;
+    g.pad[0] = g.pad[1] = g.pad[2] = 0;
+    g.graft_address = r.address;
+    records[k] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// group selection and compaction
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool selected(const svo_usage *u, uint32_t g, uint32_t pred) {
+  const uint32_t t = u[g].tick;
+  switch (pred) {
+    case kSelLive: return t != 0u;
+    case kSelFree: return t == 0u;
+    default: return t != 0u && t < 65534u;  // kSelMovable
+  }
+}
+
+constexpr uint32_t kSelChunk = 1024, kSelPer = kSelChunk / kWg;
+
+template <int WG>
+__device__ uint32_t wg_scan(uint32_t v, uint32_t *s_waves, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if ((int)lane >= d) incl += o;
+  }
+  if (lane == 63u) s_waves[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < (uint32_t)(WG / 64); w++) {
+    const uint32_t tw = s_waves[w];
+    before += w < wave ? tw : 0u;
+    all += tw;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+__global__ __launch_bounds__(kWg) void select_count_kernel(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t pred,
+                                                           uint32_t *counts) {
+  __shared__ uint32_t s_waves[kWg / 64];
+  const uint32_t g0 = lo + blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < kSelPer; j++) c += (g0 + j < hi && selected(u, g0 + j, pred)) ? 1u : 0u;
+  uint32_t all;
+  wg_scan<kWg>(c, s_waves, &all);
+  if (threadIdx.x == 0) counts[blockIdx.x] = all;
+}
+
+__global__ __launch_bounds__(kWg) void select_scatter_kernel(const svo_usage *u, uint32_t lo, uint32_t hi,
+                                                             uint32_t pred, const uint32_t *offsets, uint32_t *list) {
+  __shared__ uint32_t s_waves[kWg / 64];
+  const uint32_t g0 = lo + blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
+  bool is[kSelPer];
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < kSelPer; j++) {
+    is[j] = g0 + j < hi && selected(u, g0 + j, pred);
+    c += is[j] ? 1u : 0u;
+  }
+  uint32_t all;
+  uint32_t r = offsets[blockIdx.x] + wg_scan<kWg>(c, s_waves, &all);
+  for (uint32_t j = 0; j < kSelPer; j++)
+    if (is[j]) list[r++] = g0 + j;
+}
+
+// Exclusive ranks of the flagged entries (flags 0/1), chunk by chunk like the selection.
+__global__ __launch_bounds__(kWg) void flag_count_kernel(const uint32_t *flags, uint32_t n, uint32_t *counts) {
+  __shared__ uint32_t s_waves[kWg / 64];
+  const uint32_t i0 = blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < kSelPer; j++) c += i0 + j < n ? flags[i0 + j] : 0u;
+  uint32_t all;
+  wg_scan<kWg>(c, s_waves, &all);
+  if (threadIdx.x == 0) counts[blockIdx.x] = all;
+}
+
+__global__ __launch_bounds__(kWg) void flag_rank_kernel(const uint32_t *flags, uint32_t n, const uint32_t *offsets,
+                                                        uint32_t *rank) {
+  __shared__ uint32_t s_waves[kWg / 64];
+  const uint32_t i0 = blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
+  uint32_t f[kSelPer], c = 0;
+  for (uint32_t j = 0; j < kSelPer; j++) {
+    f[j] = i0 + j < n ? flags[i0 + j] : 0u;
+    c += f[j];
+  }
+  uint32_t all;
+  uint32_t r = offsets[blockIdx.x] + wg_scan<kWg>(c, s_waves, &all);
+  for (uint32_t j = 0; j < kSelPer; j++) {
+    if (i0 + j < n) rank[i0 + j] = r;
+    r += f[j];
+  }
+}
+
+__global__ __launch_bounds__(kWg) void remap_kernel(const uint32_t *from, const uint32_t *to, uint32_t m,
+                                                    uint32_t *remap) {
+  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+  if (i < m) remap[from[i]] = to[i];
+}
+
+__device__ __forceinline__ uint32_t remap_block(const uint32_t *remap, uint32_t b) {
+  return remap[b >> 3] * 8u + (b & 7u);
+}
+
+// Copies moved groups (blocks with remapped Child fields, usage with remapped Parent).
+__global__ __launch_bounds__(kWg) void move_kernel(PoolView p, const uint32_t *from, const uint32_t *to, uint32_t m,
+                                                   const uint32_t *remap) {
+  const uint32_t tid = blockIdx.x * kWg + threadIdx.x;
+  const uint32_t i = tid >> 3, k = tid & 7u;
+  if (i >= m) return;
+  const uint32_t o = from[i], n = to[i];
+  uint4 b = reinterpret_cast<const uint4 *>(p.blocks)[8u * o + k];
+  if (b.x != kU32Max && (b.x >> 3) < p.n_usages) b.x = remap_block(remap, b.x);
+  reinterpret_cast<uint4 *>(p.blocks)[8u * n + k] = b;
+  if (k == 0) {
+    svo_usage us = p.usages[o];
+    if (us.parent < p.n_blocks && (us.parent >> 3) < p.n_usages) us.parent = remap_block(remap, us.parent);
+    p.usages[n] = us;
+  }
+}
+
+// Links into moved groups from groups that stay, then the old groups are freed.
+__global__ __launch_bounds__(kWg) void relink_kernel(PoolView p, const uint32_t *from, const uint32_t *to, uint32_t m,
+                                                     const uint32_t *remap) {
+  const uint32_t tid = blockIdx.x * kWg + threadIdx.x;
+  const uint32_t i = tid >> 3, k = tid & 7u;
+  if (i >= m) return;
+  const uint32_t o = from[i], n = to[i];
+  if (k == 0) {
+    const uint32_t pb = p.usages[n].parent;
+    if (pb < p.n_blocks && (pb >> 3) < p.n_usages && remap[pb >> 3] == (pb >> 3)) p.blocks[pb].child = 8u * n;
+  }
+  const uint32_t c = p.blocks[8u * n + k].child;
+  if (c == kU32Max || (c >> 3) >= p.n_usages) return;
+  const uint32_t cg = c >> 3;
+  const uint32_t t = p.usages[cg].tick;
+  if (t < 65534u) {
+    // a child that stayed and points back at the old block; a moved child's copy already
+    // carries the remapped back-pointer
+    if (p.usages[cg].parent == 8u * o + k) p.usages[cg].parent = 8u * n + k;
+  } else if (t == 65534u && p.parents) {
+    uint32_t e = p.usages[cg].parent, steps = 0;
+    while (e < p.n_parents && steps++ < p.n_parents) {
+      if (p.parents[e].parent_address == 8u * o + k) p.parents[e].parent_address = 8u * n + k;
+      e = p.parents[e].next_element;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kWg) void release_moved_kernel(PoolView p, const uint32_t *from, uint32_t m) {
+  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+  if (i >= m) return;
+  p.usages[from[i]].tick = 0;
+  p.usages[from[i]].parent = kU32Max;
+}
+
+__global__ __launch_bounds__(kWg) void identity_kernel(uint32_t *remap, uint32_t n) {
+  for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += gridDim.x * kWg) remap[i] = i;
+}
+
+inline uint32_t grid_for(size_t n) { return (uint32_t)std::min<size_t>((n + kWg - 1) / kWg, 65535 * 16); }
+inline int rc_of() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+}  // namespace
+
+int launch_terrain(const TerrainView &t, hipStream_t s) {
+  const uint32_t res = 1u << t.spec.lres;
+  hipLaunchKernelGGL(terrain_top_kernel, dim3(grid_for((size_t)res * res)), dim3(kWg), 0, s, t.spec, t.top);
+  if (rc_of()) return -1;
+  for (int L = (int)t.spec.lres - 1; L >= 0; L--) {
+    const uint32_t n = 1u << L;
+    const float *imn = (uint32_t)(L + 1) == t.spec.lres ? t.top : t.hmin + t.level_offset[L + 1];
+    const float *imx = (uint32_t)(L + 1) == t.spec.lres ? t.top : t.hmax + t.level_offset[L + 1];
+    hipLaunchKernelGGL(terrain_reduce_kernel, dim3(grid_for((size_t)n * n)), dim3(kWg), 0, s, imn, imx,
+                       t.hmin + t.level_offset[L], t.hmax + t.level_offset[L], n);
+    if (rc_of()) return -1;
+  }
+  return 0;
+}
+
+int launch_seed(const TerrainView &t, uint16_t *bases, uint32_t B, uint4 *blocks, hipStream_t s) {
+  for (uint32_t d = 1; d <= B; d++) {
+    const uint32_t offset = ((1u << (3u * d)) - 1u) / 7u - 1u;  // powSum(d-1)
+    hipLaunchKernelGGL(seed_bases_kernel, dim3(grid_for(1ull << (3 * d))), dim3(kWg), 0, s, t, bases, d, offset);
+    if (rc_of()) return -1;
+  }
+  hipLaunchKernelGGL(seed_dense_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, t, bases, B, blocks);
+  return rc_of();
+}
+
+int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t mask,
+                          const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
+                          hipStream_t s) {
+  if (!n) return 0;
+  hipLaunchKernelGGL(request_first_kernel, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, s, req, n, keys, vals, mask,
+                     n_blocks);
+  if (rc_of()) return -1;
+  hipLaunchKernelGGL(request_flag_kernel, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, s, req, n, keys, vals, mask,
+                     blocks, n_blocks, max_depth, flags);
+  return rc_of();
+}
+
+int launch_bricks(const TerrainView &t, const svo_child_request *req, uint32_t n, const uint32_t *flags,
+                  const uint32_t *rank, uint32_t limit, svo_grafting *records, uint4 *bricks, hipStream_t s) {
+  if (!n || !limit) return 0;
+  hipLaunchKernelGGL(brick_kernel, dim3((uint32_t)((8ull * n + kWg - 1) / kWg)), dim3(kWg), 0, s, t, req, n, flags,
+                     rank, limit, records, bricks);
+  return rc_of();
+}
+
+int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s) {
+  const uint32_t chunks = (n + kSelChunk - 1) / kSelChunk;
+  if (!chunks) return 0;
+  hipLaunchKernelGGL(flag_count_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts);
+  if (rc_of()) return -1;
+  if (launch_scan_u32(counts, counts, chunks, total, s)) return -1;
+  hipLaunchKernelGGL(flag_rank_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts, rank);
+  return rc_of();
+}
+
+uint32_t select_chunks(uint32_t lo, uint32_t hi) { return hi > lo ? (hi - lo + kSelChunk - 1) / kSelChunk : 0u; }
+
+int launch_select_count(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t pred, uint32_t *counts, hipStream_t s) {
+  const uint32_t chunks = select_chunks(lo, hi);
+  if (!chunks) return 0;
+  hipLaunchKernelGGL(select_count_kernel, dim3(chunks), dim3(kWg), 0, s, u, lo, hi, pred, counts);
+  return rc_of();
+}
+
+int launch_select_scatter(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t pred, const uint32_t *offsets,
+                          uint32_t *list, hipStream_t s) {
+  const uint32_t chunks = select_chunks(lo, hi);
+  if (!chunks) return 0;
+  hipLaunchKernelGGL(select_scatter_kernel, dim3(chunks), dim3(kWg), 0, s, u, lo, hi, pred, offsets, list);
+  return rc_of();
+}
+
+int launch_compact_moves(const PoolView &p, const uint32_t *from, const uint32_t *to, uint32_t m, uint32_t *remap,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(identity_kernel, dim3(grid_for(p.n_usages)), dim3(kWg), 0, s, remap, p.n_usages);
+  if (rc_of() || !m) return rc_of();
+  hipLaunchKernelGGL(remap_kernel, dim3((m + kWg - 1) / kWg), dim3(kWg), 0, s, from, to, m, remap);
+  if (rc_of()) return -1;
+  hipLaunchKernelGGL(move_kernel, dim3((8ull * m + kWg - 1) / kWg), dim3(kWg), 0, s, p, from, to, m, remap);
+  if (rc_of()) return -1;
+  hipLaunchKernelGGL(relink_kernel, dim3((8ull * m + kWg - 1) / kWg), dim3(kWg), 0, s, p, from, to, m, remap);
+  if (rc_of()) return -1;
+  hipLaunchKernelGGL(release_moved_kernel, dim3((m + kWg - 1) / kWg), dim3(kWg), 0, s, p, from, m);
+  return rc_of();
+}
+
+}  // namespace svo

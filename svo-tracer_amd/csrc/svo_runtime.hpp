@@ -55,6 +55,11 @@ struct Context;
 int update_camera_table(Context *ctx, const svo_trace_input &in);
 // KernelName.Prune / KernelName.Graft behind svo_enqueue (arguments bound with SetArg).
 int enqueue_tree(Context *ctx, int kernel, uint32_t n);
+// Graft of device-resident records and graft blocks against the context's pool.
+int graft_device(Context *ctx, const svo_update_input_data &in, const svo_grafting *records, uint32_t n,
+                 const svo_block *graft_blocks, uint32_t n_graft_blocks, svo_tree_stats *stats);
+// Garbage collection's reclaim of stale live groups (age > in.offset) in the context's pool.
+int reclaim_device(Context *ctx, const svo_update_input_data &in, uint32_t *reclaimed);
 int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb);
 // A frame prepared for submission (svo_trace_frame / svo_trace_batch / captured graphs).
 struct BatchPlan {
@@ -97,6 +102,9 @@ struct Context {
   // scratch of tree management (prune / graft / collect) and its pinned counter mirror
   std::shared_ptr<DeviceBuffer> tree_scratch[24];
   uint32_t *tree_host = nullptr;
+  // scratch of streaming (request consumption, compaction) and its pinned counter mirror
+  std::shared_ptr<DeviceBuffer> stream_scratch[12];
+  uint32_t *stream_host = nullptr;
   // batched cameras: concatenated camera tables (cache key), camera records
   std::shared_ptr<DeviceBuffer> scratch_batch_lut;
   std::vector<float> batch_lut_key;

@@ -1,0 +1,54 @@
+// Launch interface of the streaming kernels: GPU terrain, seed tree, child-request
+// consumption into bricks, group selection and pool compaction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "svo.h"
+#include "svo_terrain.hpp"
+#include "svo_tree_kernels.hpp"
+
+namespace svo {
+
+constexpr uint32_t kMaxTerrainLres = 15;
+
+// Device view of a terrain: the finest height grid and its min/max pyramid.
+struct TerrainView {
+  terrain::Spec spec;
+  float *top;                                // 2^lres x 2^lres heights, row-major (j * res + i)
+  float *hmin, *hmax;                        // pyramid levels 0..lres-1, concatenated
+  size_t level_offset[kMaxTerrainLres + 1];  // element offset of level L in hmin / hmax
+};
+
+// Group predicates of the selection kernels.
+enum : uint32_t { kSelLive = 0, kSelFree = 1, kSelMovable = 2 };
+
+int launch_terrain(const TerrainView &t, hipStream_t s);
+int launch_seed(const TerrainView &t, uint16_t *bases, uint32_t B, uint4 *blocks, hipStream_t s);
+// flags[i] = 1 when request i asks for a level (TreeSize >= 1), is the first such request for
+// its block address, and the block can take a child level (address in range, childless,
+// 1 <= depth < max_depth, an octant with both air and geometry).  keys / vals are a
+// table of mask+1 entries, filled with UINT32_MAX beforehand.
+int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t mask,
+                          const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
+                          hipStream_t s);
+// Flagged request i with rank[i] < limit becomes graft record rank[i] and brick blocks
+// [8 rank[i], 8 rank[i] + 8).
+int launch_bricks(const TerrainView &t, const svo_child_request *req, uint32_t n, const uint32_t *flags,
+                  const uint32_t *rank, uint32_t limit, svo_grafting *records, uint4 *bricks, hipStream_t s);
+// rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.
+int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s);
+// Groups of [lo, hi) that satisfy a predicate, in ascending order: per-chunk counts, then the
+// scatter of the list given the scanned counts.
+uint32_t select_chunks(uint32_t lo, uint32_t hi);
+int launch_select_count(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t pred, uint32_t *counts, hipStream_t s);
+int launch_select_scatter(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t pred, const uint32_t *offsets,
+                          uint32_t *list, hipStream_t s);
+// Moves groups from[i] -> to[i] (i < m), fixing Child fields, Usage.Parent back-pointers and
+// parent-list entries; `remap` has one entry per group.
+int launch_compact_moves(const PoolView &p, const uint32_t *from, const uint32_t *to, uint32_t m, uint32_t *remap,
+                         hipStream_t s);
+
+}  // namespace svo

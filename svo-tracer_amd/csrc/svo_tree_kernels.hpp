@@ -97,13 +97,15 @@ int launch_unlink(const PoolView &p, const TreeScratch &t, hipStream_t s);
 int launch_prune_apply(const PoolView &p, const svo_pruning *rec, uint32_t n, const svo_block *surface,
                        const TreeScratch &t, hipStream_t s);
 
-// collection: one pass with `offset`; candidates ranked in ascending group order
-int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, const TreeScratch &t,
-                         hipStream_t s);
-int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, uint32_t from, uint32_t limit,
-                           const TreeScratch &t, hipStream_t s);
+// collection: one pass with `offset`; candidates ranked in ascending group order.  With
+// live_only (reclaim) only stale live groups are candidates; mark gives claimed groups `tick`.
+int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only,
+                         const TreeScratch &t, hipStream_t s);
+int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only, uint32_t from,
+                           uint32_t limit, const TreeScratch &t, hipStream_t s);
 int launch_collect_claim(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s);
-int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s);
+int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, uint32_t tick, const TreeScratch &t,
+                        hipStream_t s);
 
 // graft
 int launch_graft_sizes(const svo_grafting *rec, uint32_t n, uint32_t n_gb, const TreeScratch &t, hipStream_t s);

@@ -179,21 +179,46 @@ struct Op {
     if (in.memory_size) ng = (uint32_t)std::min<uint64_t>(((uint64_t)in.memory_size + 7u) / 8u, ng);
     uint32_t count = 0, offset = in.offset;
     while (count < need) {
-      if ((rc = launch(launch_collect_count(pv, ng, offset, ts, ctx->stream), "collect count"))) return rc;
+      if ((rc = launch(launch_collect_count(pv, ng, offset, false, ts, ctx->stream), "collect count"))) return rc;
       if ((rc = read(kTcTotal, 1))) return rc;
       const uint32_t k = std::min(host[kTcTotal], need - count);
       if (k) {
-        if ((rc = launch(launch_collect_scatter(pv, ng, offset, count, k, ts, ctx->stream), "collect scatter")))
+        if ((rc = launch(launch_collect_scatter(pv, ng, offset, false, count, k, ts, ctx->stream), "collect scatter")))
           return rc;
         if ((rc = launch(launch_collect_claim(pv, count, k, ts, ctx->stream), "collect claim"))) return rc;
         if ((rc = free_walk(true))) return rc;
-        if ((rc = launch(launch_collect_mark(pv, count, k, ts, ctx->stream), "collect mark"))) return rc;
+        if ((rc = launch(launch_collect_mark(pv, count, k, pv.now, ts, ctx->stream), "collect mark"))) return rc;
         count += k;
       }
       if (offset == 0) break;
       offset >>= 1;
     }
     *got = count;
+    return SVO_OK;
+  }
+
+  // Garbage collection's reclaim: every stale live group (age > in.offset) is cut from its
+  // parent, what hangs below it is freed and the group itself gets tick 0.
+  int reclaim(const svo_update_input_data &in, uint32_t *got) {
+    *got = 0;
+    int rc = launch(tree_fill_u32(ts.claimed, 0, ts.claimed_words + 1, ctx->stream), "fill");
+    if (rc) return rc;
+    uint32_t ng = pv.n_usages;
+    if (in.memory_size) ng = (uint32_t)std::min<uint64_t>(((uint64_t)in.memory_size + 7u) / 8u, ng);
+    if ((rc = launch(launch_collect_count(pv, ng, in.offset, true, ts, ctx->stream), "reclaim count"))) return rc;
+    if ((rc = read(kTcTotal, 1))) return rc;
+    const uint32_t k = host[kTcTotal];
+    if (!k) return SVO_OK;
+    void *p;
+    if ((rc = grow(kTbList, (size_t)k * 4, &p))) return rc;
+    ts.list = (uint32_t *)p;
+    ts.list_capacity = k;
+    if ((rc = launch(launch_collect_scatter(pv, ng, in.offset, true, 0, k, ts, ctx->stream), "reclaim scatter")))
+      return rc;
+    if ((rc = launch(launch_collect_claim(pv, 0, k, ts, ctx->stream), "reclaim claim"))) return rc;
+    if ((rc = free_walk(true))) return rc;
+    if ((rc = launch(launch_collect_mark(pv, 0, k, 0u, ts, ctx->stream), "reclaim mark"))) return rc;
+    *got = k;
     return SVO_OK;
   }
 
@@ -347,6 +372,18 @@ struct Bound {
 
 }  // namespace
 
+int graft_device(Context *ctx, const svo_update_input_data &in, const svo_grafting *records, uint32_t n,
+                 const svo_block *graft_blocks, uint32_t n_graft_blocks, svo_tree_stats *stats) {
+  return graft_impl(ctx, pool_from_context(ctx), in, records, n, graft_blocks, n_graft_blocks, nullptr, 0, stats);
+}
+
+int reclaim_device(Context *ctx, const svo_update_input_data &in, uint32_t *reclaimed) {
+  Op op{ctx};
+  int rc = op.init(pool_from_context(ctx), in);
+  if (rc) return rc;
+  return op.reclaim(in, reclaimed);
+}
+
 int enqueue_tree(Context *ctx, int kernel, uint32_t n) {
   Bound b{ctx, kernel};
   svo_update_input_data in;
@@ -406,6 +443,15 @@ extern "C" int svo_graft(svo_context *ctx, const svo_update_input_data *input, c
   if (rc) return rc;
   return graft_impl(ctx, pool_from_context(ctx), *input, (const svo_grafting *)d_rec, n, (const svo_block *)d_gb,
                     n_graft_blocks, (const uint64_t *)d_loc, n_locations, stats);
+}
+
+extern "C" int svo_reclaim(svo_context *ctx, const svo_update_input_data *input, uint32_t *reclaimed) {
+  if (!ctx || !input) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_reclaim: null argument");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(SVO_ERR_HIP, "hipSetDevice");
+  uint32_t got = 0;
+  const int rc = reclaim_device(ctx, *input, &got);
+  if (!rc && reclaimed) *reclaimed = got;
+  return rc;
 }
 
 extern "C" int svo_collect(svo_context *ctx, const svo_update_input_data *input, uint32_t need, uint32_t *groups,

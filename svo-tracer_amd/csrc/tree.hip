@@ -444,26 +444,31 @@ __global__ __launch_bounds__(kScanWg) void unlink_kernel(PoolView p, const uint2
 
 constexpr uint32_t kPerThread = kCollectChunk / kFlat;
 
+// live_only: garbage collection's reclaim, which takes stale live groups and leaves free
+// ones where they are.
 __device__ __forceinline__ bool candidate(const PoolView &p, uint32_t g, uint32_t ng, uint32_t offset,
-                                          const uint32_t *claimed) {
+                                          const uint32_t *claimed, uint32_t live_only) {
   if (g >= ng) return false;
   if ((claimed[g >> 5] >> (g & 31u)) & 1u) return false;
-  return stale(p.usages[g].tick, p.now, offset);
+  const uint32_t t = p.usages[g].tick;
+  if (live_only && t == 0u) return false;
+  return stale(t, p.now, offset);
 }
 
 __global__ __launch_bounds__(kFlat) void collect_count_kernel(PoolView p, uint32_t ng, uint32_t offset,
-                                                              const uint32_t *claimed, uint32_t *chunk_counts) {
+                                                              uint32_t live_only, const uint32_t *claimed,
+                                                              uint32_t *chunk_counts) {
   __shared__ uint32_t s_waves[kFlat / 64];
   const uint32_t g0 = blockIdx.x * kCollectChunk + threadIdx.x * kPerThread;
   uint32_t c = 0;
-  for (uint32_t j = 0; j < kPerThread; j++) c += candidate(p, g0 + j, ng, offset, claimed) ? 1u : 0u;
+  for (uint32_t j = 0; j < kPerThread; j++) c += candidate(p, g0 + j, ng, offset, claimed, live_only) ? 1u : 0u;
   uint32_t all;
   block_scan<kFlat>(c, s_waves, &all);
   if (threadIdx.x == 0) chunk_counts[blockIdx.x] = all;
 }
 
 __global__ __launch_bounds__(kFlat) void collect_scatter_kernel(PoolView p, uint32_t ng, uint32_t offset,
-                                                                const uint32_t *claimed,
+                                                                uint32_t live_only, const uint32_t *claimed,
                                                                 const uint32_t *chunk_offsets, uint32_t from,
                                                                 uint32_t limit, uint32_t *list) {
   __shared__ uint32_t s_waves[kFlat / 64];
@@ -471,7 +476,7 @@ __global__ __launch_bounds__(kFlat) void collect_scatter_kernel(PoolView p, uint
   bool is[kPerThread];
   uint32_t c = 0;
   for (uint32_t j = 0; j < kPerThread; j++) {
-    is[j] = candidate(p, g0 + j, ng, offset, claimed);
+    is[j] = candidate(p, g0 + j, ng, offset, claimed, live_only);
     c += is[j] ? 1u : 0u;
   }
   uint32_t all;
@@ -503,12 +508,13 @@ __global__ __launch_bounds__(kFlat) void collect_claim_kernel(PoolView p, const 
   }
 }
 
+// Claimed groups get `tick`: the frame tick when collected for a graft, 0 when reclaimed.
 __global__ __launch_bounds__(kFlat) void collect_mark_kernel(PoolView p, const uint32_t *list, uint32_t from,
-                                                             uint32_t k) {
+                                                             uint32_t k, uint32_t tick) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
   if (i >= k) return;
   const uint32_t g = list[from + i];
-  p.usages[g].tick = (uint16_t)p.now;
+  p.usages[g].tick = (uint16_t)tick;
   p.usages[g].parent = kU32Max;
 }
 
@@ -752,22 +758,22 @@ int launch_prune_apply(const PoolView &p, const svo_pruning *rec, uint32_t n, co
   return launch_rc();
 }
 
-int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, const TreeScratch &t,
-                         hipStream_t s) {
+int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only,
+                         const TreeScratch &t, hipStream_t s) {
   const uint32_t chunks = (n_groups + kCollectChunk - 1u) / kCollectChunk;
   if (!chunks) return 0;
-  hipLaunchKernelGGL(collect_count_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset, t.claimed,
-                     t.chunk_counts);
+  hipLaunchKernelGGL(collect_count_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset,
+                     live_only ? 1u : 0u, t.claimed, t.chunk_counts);
   if (launch_rc()) return -1;
   return launch_scan_u32(t.chunk_counts, t.chunk_counts, chunks, &t.counters[kTcTotal], s);
 }
 
-int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, uint32_t from, uint32_t limit,
-                           const TreeScratch &t, hipStream_t s) {
+int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only, uint32_t from,
+                           uint32_t limit, const TreeScratch &t, hipStream_t s) {
   const uint32_t chunks = (n_groups + kCollectChunk - 1u) / kCollectChunk;
   if (!chunks || !limit) return 0;
-  hipLaunchKernelGGL(collect_scatter_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset, t.claimed,
-                     t.chunk_counts, from, limit, t.list);
+  hipLaunchKernelGGL(collect_scatter_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset,
+                     live_only ? 1u : 0u, t.claimed, t.chunk_counts, from, limit, t.list);
   return launch_rc();
 }
 
@@ -778,9 +784,10 @@ int launch_collect_claim(const PoolView &p, uint32_t from, uint32_t k, const Tre
   return launch_rc();
 }
 
-int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s) {
+int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, uint32_t tick, const TreeScratch &t,
+                        hipStream_t s) {
   if (!k) return 0;
-  hipLaunchKernelGGL(collect_mark_kernel, dim3(blocks_for(k)), dim3(kFlat), 0, s, p, t.list, from, k);
+  hipLaunchKernelGGL(collect_mark_kernel, dim3(blocks_for(k)), dim3(kFlat), 0, s, p, t.list, from, k, tick);
   return launch_rc();
 }
 

@@ -8,6 +8,8 @@
   depth 8, world [-1.25, 1.25]^3 mapped onto the octree's unit cube;
 * ``config2``: 1920x1080, depth-12 fBm heightfield (6 octaves, lacunarity 2,
   gain 0.5, seed 7) over a 4096^2 grid, 100 frames along a seeded orbit;
+* ``config3``: the same heightfield law on a 16384^2 grid, streamed to depth 14 along a
+  seeded 300-frame spline fly-through at 3840x2160 (seed 3);
 * ``config4``: 64 cameras around the config-2 scene, resolutions log-uniform between
   256x256 and 1920x1080, poses on a hemisphere, apertures 1-4 px (seed 11);
 * ``config5``: depth-10 sphere+noise scene (seed 3), 32 views of 512x512.
@@ -180,6 +182,57 @@ def config2_camera(frame: int, width: int = 1920, height: int = 1080, frames: in
     fov_x = math.radians(60.0)
     t = (frame % 65533) + 1 if tick is None else tick
     return trace_input(eye, look_at(eye, target), width, height, fov_x, fov_x * height / width, 4, t)
+
+
+def config3_geometry(resolution: int = 16384, seed: int = 7) -> Geometry:
+    """The streamed terrain: the config-2 fBm heightfield law (6 octaves, lacunarity 2,
+    gain 0.5) sampled on a 16384^2 grid, so that depth-14 cells have their own samples."""
+    g = Geometry()
+    g.kind = GEOMETRY_FBM_HEIGHTFIELD
+    g.seed = seed
+    g.octaves = 6
+    g.lacunarity = 2.0
+    g.gain = 0.5
+    g.frequency = 4.0
+    g.resolution = resolution
+    g.z_min = 0.36
+    g.z_max = 0.44
+    return g
+
+
+def _catmull_rom(p0, p1, p2, p3, t):
+    t2, t3 = t * t, t * t * t
+    return [0.5 * (2 * b + (-a + c) * t + (2 * a - 5 * b + 4 * c - d) * t2 + (-a + 3 * b - 3 * c + d) * t3)
+            for a, b, c, d in zip(p0, p1, p2, p3)]
+
+
+def config3_camera(frame: int, width: int = 3840, height: int = 2160, frames: int = 300, tick: int = None,
+                   seed: int = 3, control_points: int = 8, max_child_request_id: int = 0,
+                   base_depth: int = 4) -> TraceInput:
+    """Frame `frame` of the seeded fly-through: a closed Catmull-Rom spline through
+    `control_points` seeded points 0.05-0.09 above the terrain's top (z 0.49-0.53) over
+    [0.15, 0.85]^2, travelled once in `frames` frames; the camera looks along the path,
+    pitched 25 degrees down."""
+    rng = SplitMix64(seed)
+    pts = [(0.15 + 0.7 * rng.uniform(), 0.15 + 0.7 * rng.uniform(), 0.49 + 0.04 * rng.uniform())
+           for _ in range(control_points)]
+
+    def at(u):
+        i = int(math.floor(u))
+        return _catmull_rom(*[pts[(i + k - 1) % control_points] for k in range(4)], u - i)
+
+    u = (frame % frames) / frames * control_points
+    eye, ahead = at(u), at(u + 1e-3)
+    dx, dy = ahead[0] - eye[0], ahead[1] - eye[1]
+    norm = math.hypot(dx, dy) or 1.0
+    pitch = math.radians(25.0)
+    fwd = (math.cos(pitch) * dx / norm, math.cos(pitch) * dy / norm, -math.sin(pitch))
+    target = (eye[0] + fwd[0], eye[1] + fwd[1], eye[2] + fwd[2])
+    fov_x = math.radians(60.0)
+    tk = (frame % 65533) + 1 if tick is None else tick
+    cam = trace_input(eye, look_at(eye, target), width, height, fov_x, fov_x * height / width, base_depth, tk)
+    cam.max_child_request_id = max_child_request_id or width * height
+    return cam
 
 
 def config4_cameras(count: int = 64, tick: int = 1, seed: int = 11) -> Tuple[List[TraceInput], List[float]]:

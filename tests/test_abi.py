@@ -69,6 +69,12 @@ def test_errors_raise_with_method_and_code(svo):
     assert lib.svo_prune(None, None, None, 0, None, 0, None, 0, None) == -1
     assert lib.svo_graft(None, None, None, 0, None, 0, None, 0, None) == -1
     assert lib.svo_collect(None, None, 0, None, None) == -1
+    assert lib.svo_reclaim(None, None, None) == -1
+    assert lib.svo_compact(None, None, None) == -1
+    assert lib.svo_terrain_create(None, None, None) == -1
+    assert lib.svo_stream_seed(None, None, 4) == -1
+    assert lib.svo_stream_update(None, None, None, None) == -1
+    assert lib.svo_terrain_destroy(None) == 0
 
 
 def test_create_without_device_fails_cleanly(svo):

@@ -1,0 +1,201 @@
+"""GPU parity of streaming (svo_terrain_create, svo_stream_seed, svo_stream_update, svo_reclaim,
+svo_compact) with the oracle: the seed equals TreeBuilder's dense level, every brick equals the
+statically built tree's children of its cell, and after every frame the traced outputs, the
+requests, the graft records and the whole pool are bit-exact against the oracle's streaming
+step run on its own copy of the pool."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from frame_helpers import first_mismatch
+from stream_helpers import U32MAX, pow_sum, seed_pool, static_children, static_tree, terrain_geometry
+from tree_helpers import check_pool
+
+pytestmark = pytest.mark.gpu
+
+
+class GpuStream:
+    """A streamed pool on the device: terrain, seed, per-frame trace and update."""
+
+    def __init__(self, svo, cm, geom, B: int, n_groups: int, width: int, height: int):
+        self.svo, self.cm, self.B = svo, cm, B
+        self.n = width * height
+        N = svo.BufferName
+        for name in (N.Parents, N.ParentResidency, N.VisitHashes, N.BackgroundQueue, N.MaterialQueue):
+            try:
+                cm.ReleaseBuffer(name)
+            except svo.SvoError:
+                pass
+        cm.InitDeviceBuffer(N.Bases, pow_sum(B), 2)
+        cm.InitDeviceBuffer(N.Blocks, 8 * n_groups, 16)
+        cm.InitDeviceBuffer(N.Usages, n_groups, 8)
+        cm.InitDeviceBuffer(N.ChildRequests, self.n, 40)
+        cm.InitBuffer(N.ChildRequestID, np.zeros(1, np.uint32))
+        cm.InitDeviceBuffer(N.OutputImage, self.n, 16)
+        cm.InitDeviceBuffer(N.HitIndices, self.n, 4)
+        cm.InitDeviceBuffer(N.VisitCounts, self.n, 4)
+        cm.InitDeviceBuffer(N.RayStatus, self.n, 1)
+        cm.InitDeviceBuffer(N.Locations, self.n, 32)
+        cm.InitDeviceBuffer(N.Depths, self.n, 1)
+        self.terrain = cm.CreateTerrain(geom)
+        cm.StreamSeed(self.terrain, B)
+
+    def pool(self):
+        N, cm, svo = self.svo.BufferName, self.cm, self.svo
+        return (cm.ReadBuffer(N.Bases, np.uint16), cm.ReadBuffer(N.Blocks, svo.BLOCK_DTYPE),
+                cm.ReadBuffer(N.Usages, svo.USAGE_DTYPE))
+
+    def trace(self, cam, tick: int) -> dict:
+        N, cm, svo, n = self.svo.BufferName, self.cm, self.svo, self.n
+        cam.tick = tick
+        cam.max_child_request_id = n
+        cm.TraceFrame(cam, flags=svo.FRAME_STATE)
+        cm.Wait()
+        count = int(cm.ReadBuffer(N.ChildRequestID, np.uint32, 1)[0])
+        return {
+            "rgba": cm.ReadBuffer(N.OutputImage, np.float32, 4 * n).reshape(n, 4),
+            "hit": cm.ReadBuffer(N.HitIndices, np.uint32, n),
+            "visits": cm.ReadBuffer(N.VisitCounts, np.uint32, n),
+            "status": cm.ReadBuffer(N.RayStatus, np.uint8, n),
+            "depth": cm.ReadBuffer(N.Depths, np.uint8, n),
+            "location": cm.ReadBuffer(N.Locations, np.uint64, 4 * n).reshape(n, 4)[:, :3],
+            "request_count": count,
+            "requests": cm.ReadBuffer(N.ChildRequests, svo.CHILD_REQUEST_DTYPE, min(count, n)),
+        }
+
+    def update(self, tick: int, max_depth: int, **kw) -> dict:
+        N, cm, svo = self.svo.BufferName, self.cm, self.svo
+        st = cm.StreamUpdate(self.terrain, self.B, max_depth, tick, max_child_request_id=self.n, **kw)
+        nb = st["bricks"]
+        st["records"] = cm.ReadBuffer(N.Grafting, svo.GRAFTING_DTYPE, nb) if nb else np.zeros(0, svo.GRAFTING_DTYPE)
+        st["bricks_array"] = (cm.ReadBuffer(N.GraftingBlocks, svo.BLOCK_DTYPE, 8 * nb) if nb
+                              else np.zeros(0, svo.BLOCK_DTYPE))
+        return st
+
+    def close(self):
+        self.terrain.close()
+
+
+def _eq(a, b, what):
+    m = first_mismatch(a, b)
+    assert m is None, f"{what}: {m}"
+
+
+@pytest.mark.parametrize("B,resolution", [(2, 64), (3, 256)])
+def test_seed_matches_the_builder(svo, gpu, B, resolution):
+    geom = terrain_geometry(svo, resolution)
+    n_groups = 8 ** (B + 1) + 100
+    gs = GpuStream(svo, gpu, geom, B, n_groups, 8, 8)
+    try:
+        bases, blocks, usages = gs.pool()
+        rb, rk, ru = seed_pool(svo, geom, B, n_groups)
+        _eq(bases, rb, "bases")
+        _eq(blocks, rk, "blocks")
+        _eq(usages, ru, "usages")
+        h = gs.terrain.heights()
+        assert h.shape == (resolution, resolution) and np.all((h >= 0.36) & (h <= 0.44)) and h.std() > 1e-3
+    finally:
+        gs.close()
+
+
+def _stream_and_compare(svo, orc, gpu, geom, B, D, n_groups, cams, hold, frames, **kw):
+    W, H = cams[0].screen_size[0], cams[0].screen_size[1]
+    st = static_tree(svo, geom, B, D)
+    gs = GpuStream(svo, gpu, geom, B, n_groups, W, H)
+    try:
+        bases, blocks, usages = gs.pool()
+        pool = orc.TreePool(bases, blocks, usages)
+        seq = list(cams) + [hold] * frames
+        history = []
+        for f, cam in enumerate(seq):
+            tick = f + 1
+            got = gs.trace(cam, tick)
+            cam.tick = tick
+            ref = orc.trace_frame(cam, pool.bases, pool.blocks, pool.usages, request_capacity=W * H, threads=8)
+            for key in ("hit", "visits", "status", "rgba", "depth", "location"):
+                _eq(got[key], ref[key], f"frame {f}: {key}")
+            assert got["request_count"] == ref["request_count"], f"frame {f}: request count"
+            _eq(got["requests"], ref["requests"], f"frame {f}: requests")
+            upd = gs.update(tick, D, **kw)
+            exp = orc.stream_update(pool, ref["requests"], ref["request_count"], W * H, B, D, tick,
+                                    lambda sel: static_children(st.blocks, B, sel), **kw)
+            for key in ("requests", "unique", "bricks", "collected", "gc_runs", "reclaimed", "moved", "free_groups"):
+                assert upd[key] == exp[key], f"frame {f}: {key} gpu={upd[key]} oracle={exp[key]}"
+            _eq(upd["records"], exp["records"].view(upd["records"].dtype), f"frame {f}: graft records")
+            _eq(upd["bricks_array"], exp["bricks_array"], f"frame {f}: bricks")
+            _, gblocks, gusages = gs.pool()
+            _eq(gusages, pool.usages, f"frame {f}: usages")
+            _eq(gblocks, pool.blocks, f"frame {f}: blocks")
+            history.append(upd)
+            if f >= len(cams) and upd["unique"] == 0:
+                break
+        check_pool(pool.blocks, pool.usages, B)
+        return history, got, st
+    finally:
+        gs.close()
+
+
+def test_stream_matches_oracle_frame_by_frame(svo, orc, gpu):
+    """Fly 10 frames, then hold: bricks, grafts, reclaim and compaction all bit-exact."""
+    geom = terrain_geometry(svo, 64)
+    B, D = 2, 7
+    cams = [svo.scenes.config3_camera(f, 48, 32, frames=10, base_depth=B) for f in range(10)]
+    hold = svo.scenes.config3_camera(3, 48, 32, frames=10, base_depth=B)
+    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 560, cams, hold, 30,
+                                        max_bricks=40, prune_age=2, gc_free_percent=50)
+    assert sum(h["bricks"] for h in history) > 200
+    assert sum(h["reclaimed"] for h in history) > 0 and sum(h["moved"] for h in history) > 0
+    assert history[-1]["unique"] == 0
+
+
+def test_stream_brick_cap_and_forced_gc(svo, orc, gpu):
+    """Few bricks per frame, garbage collection forced every frame, a deeper tree."""
+    geom = terrain_geometry(svo, 128)
+    B, D = 2, 8
+    cams = [svo.scenes.config3_camera(f, 40, 24, frames=6, base_depth=B) for f in range(6)]
+    hold = svo.scenes.config3_camera(5, 40, 24, frames=6, base_depth=B)
+    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 2048, cams, hold, 4,
+                                        max_bricks=7, prune_age=1, gc_free_percent=0,
+                                        flags=svo.STREAM_FORCE_GC)
+    assert all(h["gc_runs"] == 1 for h in history) and all(h["bricks"] <= 7 for h in history)
+
+
+def test_streamed_frame_converges_to_the_static_tree_gpu(svo, gpu):
+    """Larger: a 640x360 camera over a 1024^2 terrain streamed to depth 10 until nothing more
+    is served; the frame then equals the statically built tree's frame (addresses aside)."""
+    geom = terrain_geometry(svo, 1024)
+    B, D = 3, 10
+    cam = svo.scenes.config3_camera(100, 640, 360, base_depth=B)
+    n = 640 * 360
+    gs = GpuStream(svo, gpu, geom, B, 1 << 18, 640, 360)
+    try:
+        history = []
+        for f in range(2 * D):
+            got = gs.trace(cam, f + 1)
+            upd = gs.update(f + 1, D, max_bricks=1 << 20)
+            history.append((upd["unique"], upd["bricks"]))
+            if upd["unique"] == 0:
+                break
+        assert upd["unique"] == 0 and f > 2, history
+        st = static_tree(svo, geom, B, D)
+    finally:
+        gs.close()
+    N = svo.BufferName
+    gpu.InitBuffer(N.Bases, st.bases)
+    gpu.InitBuffer(N.Blocks, st.blocks)
+    gpu.ReleaseBuffer(N.Usages)
+    gpu.InitDeviceBuffer(N.ChildRequests, n, 40)
+    cam.max_child_request_id = n
+    gpu.TraceFrame(cam, flags=svo.FRAME_STATE)
+    gpu.Wait()
+    ref = {
+        "rgba": gpu.ReadBuffer(N.OutputImage, np.float32, 4 * n).reshape(n, 4),
+        "visits": gpu.ReadBuffer(N.VisitCounts, np.uint32, n),
+        "status": gpu.ReadBuffer(N.RayStatus, np.uint8, n),
+        "depth": gpu.ReadBuffer(N.Depths, np.uint8, n),
+    }
+    for key in ("rgba", "visits", "depth"):
+        _eq(got[key], ref[key], key)
+    _eq(got["status"] & 0xFD, ref["status"] & 0xFD, "status")
+    assert np.count_nonzero(got["hit"] != U32MAX) > n // 4
