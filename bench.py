@@ -244,9 +244,11 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
         cm.InitDeviceBuffer(B.HitIndices, n, 4)
         cm.InitDeviceBuffer(B.VisitCounts, n, 4)
         terrain = cm.CreateTerrain(svo.scenes.config3_geometry())
+        log(f"stream: terrain 16384^2 generated in {time.perf_counter() - t0:.2f} s")
         cm.StreamSeed(terrain, Bd)
         cm.Wait()
         setup_s = time.perf_counter() - t0
+        log(f"stream: seed tree ready after {setup_s:.2f} s")
         cams = [svo.scenes.config3_camera(f, width, height, frames=frames, base_depth=Bd) for f in range(frames)]
         trace_ms, update_ms, stats = [], [], []
         wall0 = time.perf_counter()
@@ -260,11 +262,23 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
             stats.append(cm.StreamUpdate(terrain, Bd, D, f + 1, max_bricks=65536, max_child_request_id=n,
                                          prune_age=30, gc_free_percent=10))
             update_ms.append((time.perf_counter() - t1) * 1e3)
-            if f % 50 == 49:
-                log(f"stream frame {f + 1}: trace {np.mean(trace_ms[-50:]):.2f} ms, update "
-                    f"{np.mean(update_ms[-50:]):.2f} ms, free groups {stats[-1]['free_groups']}")
+            if f < 3 or f % 25 == 24:
+                log(f"stream frame {f + 1}: trace {trace_ms[-1]:.2f} ms, update {update_ms[-1]:.2f} ms, "
+                    f"{stats[-1]['requests']} requests, {stats[-1]['bricks']} bricks, "
+                    f"free groups {stats[-1]['free_groups']}")
         cm.Wait()
         wall_s = time.perf_counter() - wall0
+        # one garbage collection of the final pool, timed on its own: reclaim of groups unused
+        # for 30 frames, then compaction
+        ui = svo.UpdateInputData()
+        ui.base_depth, ui.tick, ui.offset = Bd, frames, 30
+        t1 = time.perf_counter()
+        reclaimed = cm.Reclaim(ui)
+        t2 = time.perf_counter()
+        moved = cm.Compact(ui)
+        t3 = time.perf_counter()
+        log(f"stream: final garbage collection reclaimed {reclaimed} groups in {(t2 - t1) * 1e3:.2f} ms, "
+            f"compaction moved {moved} in {(t3 - t2) * 1e3:.2f} ms")
         terrain.close()
     finally:
         cm.Dispose()
@@ -287,6 +301,8 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
         "reclaimed_total": int(sum(s["reclaimed"] for s in stats)),
         "moved_total": int(sum(s["moved"] for s in stats)),
         "free_groups_last": int(stats[-1]["free_groups"]),
+        "final_gc": {"reclaimed": reclaimed, "reclaim_ms": round((t2 - t1) * 1e3, 3), "moved": moved,
+                     "compact_ms": round((t3 - t2) * 1e3, 3)},
         "pool_groups": n_groups,
         "setup_s": round(setup_s, 2),
     }

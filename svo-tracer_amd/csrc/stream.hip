@@ -156,12 +156,19 @@ __global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_reque
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
   if (i >= n) return;
   const svo_child_request r = req[i];
+  // requests request_first_kernel never inserted are not looked up
+  if (r.address >= n_blocks || r.tree_size == 0) {
+    flags[i] = 0u;
+    return;
+  }
   uint32_t h = addr_hash(r.address, mask), first = kU32Max;
   for (uint32_t probe = 0; probe <= mask; probe++) {
-    if (keys[h] == r.address) {
+    const uint32_t k = keys[h];
+    if (k == r.address) {
       first = vals[h];
       break;
     }
+    if (k == kU32Max) break;  // an empty slot ends the probe sequence
     h = (h + 1u) & mask;
   }
   bool ok = first == i && r.address < n_blocks && (uint32_t)r.depth + 1u <= max_depth && r.depth >= 1;
