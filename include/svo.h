@@ -330,7 +330,7 @@ int svo_frame_graph_destroy(svo_context *ctx, svo_frame_graph *graph);
 
 /* Picks svo_frame_options for a (batched) frame by timing `frames` real frames per
  * candidate (coordinate descent over workgroup_size 64..512, work_chunk 64..1024 cones per
- * claim and refill_lanes 16..64), starting from *base (may be NULL).  All candidates give
+ * claim, refill_lanes 16..64 and stack_levels 4..24), starting from *base (may be NULL).  All candidates give
  * the same results, and repeating a frame leaves the same state, so the caller's frame can
  * be tuned in place.  No reference counterpart. */
 int svo_autotune(svo_context *ctx, const svo_trace_input *inputs, const float *apertures, uint32_t n_cameras,

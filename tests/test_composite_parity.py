@@ -51,6 +51,11 @@ def _bind(svo, cm, bases, blocks, params, n, target=None):
         t4 = np.zeros((n, 4), np.float32)
         t4[:, :3] = target
         cm.InitBuffer(B.TargetImage, t4)
+    else:
+        try:
+            cm.ReleaseBuffer(B.TargetImage)
+        except svo.SvoError:
+            pass
 
 
 def test_layers_and_image_match_oracle(svo, orc, gpu, setup):
@@ -156,3 +161,68 @@ def test_fitting_reduces_the_loss(svo, orc, gpu, setup):
         losses.append(gpu.Composite(views, max_layers=K, adjoint=True))
         gpu.AdamStep(step, learning_rate=0.02)
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+@pytest.fixture(scope="module")
+def config5(svo, orc):
+    """The depth-10 sphere+noise scene (seed 3) and its 32 views of 512x512, with the FP64
+    layer lists of every cone."""
+    spec = svo.scenes.config5_scene()
+    tree, bases, blocks = tree_arrays(svo, spec)
+    views = svo.scenes.config5_cameras()
+    for v in views:
+        v.tick = 1
+    layers, bg = orc.composite_layers(views, bases, blocks, K, threads=16)
+    return tree, bases, blocks, views, layers, bg
+
+
+def test_adjoint_at_full_size_on_1000_leaves(svo, orc, gpu, config5):
+    """32 views of 512^2 over the depth-10 scene: targets rendered from the true leaves,
+    leaves perturbed by N(0, 0.1) (colour) and N(0, 0.05) (opacity); the device gradient
+    against the FP64 adjoint (relative error <= 1e-3) and, on 1000 random composited leaves,
+    against central differences with h = 1e-3."""
+    tree, bases, blocks, views, layers, bg = config5
+    n = layers.shape[0]
+    truth = _params_from_blocks(blocks)
+    params = _perturbed(truth, 3)
+    B = svo.BufferName
+    _bind(svo, gpu, bases, blocks, truth, n)
+    gpu.Composite(views, max_layers=K, layers=True)
+    gpu.Wait()
+    assert np.array_equal(gpu.ReadBuffer(B.Layers, np.uint32, n * K).reshape(n, K), layers)
+    target = gpu.ReadBuffer(B.OutputImage, np.float32, 4 * n).reshape(n, 4)
+    _, ref_image, _ = orc.composite_loss(layers, bg, truth.astype(np.float64), want_image=True)
+    assert np.max(np.abs(target[:, :3] - ref_image)) <= 1e-5
+    _bind(svo, gpu, bases, blocks, params, n, target=target[:, :3])
+    loss = gpu.Composite(views, max_layers=K, adjoint=True)
+    gpu.Wait()
+    got = gpu.ReadBuffer(B.LeafGradients, np.float32, 4 * blocks.size).reshape(-1, 4).astype(np.float64)
+    p64 = params.astype(np.float64)
+    t64 = target[:, :3].astype(np.float64)
+    ref_loss, _, ref = orc.composite_loss(layers, bg, p64, target=t64, want_grads=True)
+    assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss)
+    used = np.unique(layers[layers != U32MAX])
+    assert np.linalg.norm(got[used] - ref[used]) <= 1e-3 * np.linalg.norm(ref[used])
+    # the cones that composite each leaf, from one sort of the layer table
+    flat = layers.reshape(-1)
+    order = np.argsort(flat, kind="stable")
+    sorted_leaves = flat[order]
+    rng = np.random.default_rng(11)
+    pick = rng.choice(used, size=1000, replace=False)
+    floor = 1e-2 * np.max(np.abs(ref[used]))
+    h = 1e-3
+    work = p64.copy()
+    for a in pick:
+        lo, hi = np.searchsorted(sorted_leaves, [a, a + 1])
+        rays = np.unique(order[lo:hi] // K).astype(np.uint32)
+        for ch in range(4):
+            x = work[a, ch]
+            work[a, ch] = x + h
+            l1 = orc.composite_loss(layers, bg, work, target=t64, rays=rays)[0]
+            work[a, ch] = x - h
+            l0 = orc.composite_loss(layers, bg, work, target=t64, rays=rays)[0]
+            work[a, ch] = x
+            fd = (l1 - l0) / (2 * h)
+            scale = max(abs(ref[a, ch]), floor)
+            assert abs(ref[a, ch] - fd) <= 1e-3 * scale, ("fp64 adjoint vs differences", int(a), ch, ref[a, ch], fd)
+            assert abs(got[a, ch] - ref[a, ch]) <= 1e-3 * scale, ("device vs fp64", int(a), ch, got[a, ch], ref[a, ch])
