@@ -118,6 +118,49 @@ namespace SvoTracer.Kernel.Native
 		public uint Flags, MaxTreeDepth, WorkgroupSize, RefillLanes, StackLevels, WorkChunk, WavesPerSimd, Reserved;
 	}
 
+	[StructLayout(LayoutKind.Sequential)]
+	public struct NativeCompositeOptions
+	{
+		public uint MaxLayers, Flags, Reserved0, Reserved1;
+	}
+
+	[StructLayout(LayoutKind.Sequential)]
+	public struct NativeAdamOptions
+	{
+		public float LearningRate, Beta1, Beta2, Epsilon;
+		public uint Step;
+	}
+
+	[StructLayout(LayoutKind.Explicit, Size = 32)]
+	public struct NativeStreamOptions
+	{
+		[FieldOffset(0)] public uint BaseDepth;
+		[FieldOffset(4)] public uint MaxDepth;
+		[FieldOffset(8)] public ushort Tick;
+		[FieldOffset(12)] public uint MaxBricks;
+		[FieldOffset(16)] public uint MaxChildRequestId;
+		[FieldOffset(20)] public uint PruneAge;
+		[FieldOffset(24)] public uint GcFreePercent;
+		[FieldOffset(28)] public uint Flags;
+	}
+
+	[StructLayout(LayoutKind.Sequential)]
+	public struct NativeStreamStats
+	{
+		public uint Requests, Unique, Bricks, Collected, GcRuns, Reclaimed, Moved, FreeGroups, PoolGroups, Reserved0, Reserved1, Reserved2;
+	}
+
+	/// <summary>svo_geometry: scene description of the host builder and the device terrain.</summary>
+	[StructLayout(LayoutKind.Sequential)]
+	public unsafe struct NativeGeometry
+	{
+		public uint Kind, Count;
+		public float* Shapes;
+		public ulong Seed;
+		public uint Octaves, Resolution;
+		public float Lacunarity, Gain, Amplitude, Frequency, HalfExtent, ZMin, ZMax;
+	}
+
 	/// <summary>Raw entry points of include/svo.h.</summary>
 	public static unsafe class SvoNative
 	{
@@ -153,6 +196,26 @@ namespace SvoTracer.Kernel.Native
 		[DllImport(Lib)] public static extern int svo_graft(IntPtr ctx, ref NativeUpdateInputData input, NativeGrafting* records, uint n,
 			NativeBlock* graftBlocks, uint nGraftBlocks, ulong* locations, uint nLocations, out NativeTreeStats stats);
 		[DllImport(Lib)] public static extern int svo_collect(IntPtr ctx, ref NativeUpdateInputData input, uint need, uint* groups, out uint count);
+		[DllImport(Lib)] public static extern int svo_reclaim(IntPtr ctx, ref NativeUpdateInputData input, out uint reclaimed);
+		[DllImport(Lib)] public static extern int svo_compact(IntPtr ctx, ref NativeUpdateInputData input, out uint moved);
+
+		[DllImport(Lib)] public static extern int svo_trace_batch(IntPtr ctx, NativeTraceInput* inputs, float* apertures, uint nCameras,
+			ref NativeFrameOptions options);
+		[DllImport(Lib)] public static extern int svo_frame_graph_create(IntPtr ctx, NativeTraceInput* inputs, float* apertures, uint nCameras,
+			ref NativeFrameOptions options, uint perCamera, out IntPtr graph);
+		[DllImport(Lib)] public static extern int svo_frame_graph_launch(IntPtr ctx, IntPtr graph, NativeTraceInput* inputs);
+		[DllImport(Lib)] public static extern int svo_frame_graph_destroy(IntPtr ctx, IntPtr graph);
+		[DllImport(Lib)] public static extern int svo_autotune(IntPtr ctx, NativeTraceInput* inputs, float* apertures, uint nCameras, uint frames,
+			ref NativeFrameOptions baseOptions, out NativeFrameOptions best, out float bestMs);
+		[DllImport(Lib)] public static extern int svo_composite(IntPtr ctx, NativeTraceInput* views, float* apertures, uint nViews,
+			ref NativeCompositeOptions options, out double loss);
+		[DllImport(Lib)] public static extern int svo_adam_step(IntPtr ctx, ref NativeAdamOptions options);
+
+		[DllImport(Lib)] public static extern int svo_terrain_create(IntPtr ctx, ref NativeGeometry geometry, out IntPtr terrain);
+		[DllImport(Lib)] public static extern int svo_terrain_destroy(IntPtr terrain);
+		[DllImport(Lib)] public static extern int svo_stream_seed(IntPtr ctx, IntPtr terrain, uint baseDepth);
+		[DllImport(Lib)] public static extern int svo_stream_update(IntPtr ctx, IntPtr terrain, ref NativeStreamOptions options,
+			out NativeStreamStats stats);
 	}
 
 	/// <summary>Stands in for CLEvent: the library's stream is in order, so events only mark calls.</summary>
@@ -306,6 +369,84 @@ namespace SvoTracer.Kernel.Native
 					l, (uint)((locations?.Length ?? 0) / 3), out NativeTreeStats st), "Graft");
 				return st;
 			}
+		}
+
+		public uint Reclaim(NativeUpdateInputData input)
+		{
+			HandleResultCode(SvoNative.svo_reclaim(_ctx, ref input, out uint n), "Reclaim");
+			return n;
+		}
+
+		public uint Compact(NativeUpdateInputData input)
+		{
+			HandleResultCode(SvoNative.svo_compact(_ctx, ref input, out uint n), "Compact");
+			return n;
+		}
+
+		// ---- batched cameras, captured frames, tuning ----
+		public void TraceBatch(NativeTraceInput[] inputs, float[] apertures, NativeFrameOptions options = default)
+		{
+			fixed (NativeTraceInput* i = inputs) fixed (float* a = apertures)
+				HandleResultCode(SvoNative.svo_trace_batch(_ctx, i, a, (uint)inputs.Length, ref options), "TraceBatch");
+		}
+
+		public IntPtr CaptureFrame(NativeTraceInput[] inputs, float[] apertures, NativeFrameOptions options = default, bool perCamera = false)
+		{
+			fixed (NativeTraceInput* i = inputs) fixed (float* a = apertures)
+			{
+				HandleResultCode(SvoNative.svo_frame_graph_create(_ctx, i, a, (uint)inputs.Length, ref options, perCamera ? 1u : 0u,
+					out IntPtr graph), "CaptureFrame");
+				return graph;
+			}
+		}
+
+		public void LaunchFrame(IntPtr graph, NativeTraceInput[] inputs)
+		{
+			fixed (NativeTraceInput* i = inputs)
+				HandleResultCode(SvoNative.svo_frame_graph_launch(_ctx, graph, i), "LaunchFrame");
+		}
+
+		public void ReleaseFrame(IntPtr graph) => HandleResultCode(SvoNative.svo_frame_graph_destroy(_ctx, graph), "ReleaseFrame");
+
+		public (NativeFrameOptions, float) Autotune(NativeTraceInput[] inputs, float[] apertures, uint frames = 3,
+			NativeFrameOptions baseOptions = default)
+		{
+			fixed (NativeTraceInput* i = inputs) fixed (float* a = apertures)
+			{
+				HandleResultCode(SvoNative.svo_autotune(_ctx, i, a, (uint)inputs.Length, frames, ref baseOptions,
+					out NativeFrameOptions best, out float ms), "Autotune");
+				return (best, ms);
+			}
+		}
+
+		// ---- differentiable compositing ----
+		public double Composite(NativeTraceInput[] views, float[] apertures, NativeCompositeOptions options)
+		{
+			fixed (NativeTraceInput* v = views) fixed (float* a = apertures)
+			{
+				HandleResultCode(SvoNative.svo_composite(_ctx, v, a, (uint)views.Length, ref options, out double loss), "Composite");
+				return loss;
+			}
+		}
+
+		public void AdamStep(NativeAdamOptions options) => HandleResultCode(SvoNative.svo_adam_step(_ctx, ref options), "AdamStep");
+
+		// ---- streaming ----
+		public IntPtr CreateTerrain(NativeGeometry geometry)
+		{
+			HandleResultCode(SvoNative.svo_terrain_create(_ctx, ref geometry, out IntPtr terrain), "CreateTerrain");
+			return terrain;
+		}
+
+		public void DestroyTerrain(IntPtr terrain) => HandleResultCode(SvoNative.svo_terrain_destroy(terrain), "DestroyTerrain");
+
+		public void StreamSeed(IntPtr terrain, uint baseDepth)
+			=> HandleResultCode(SvoNative.svo_stream_seed(_ctx, terrain, baseDepth), "StreamSeed");
+
+		public NativeStreamStats StreamUpdate(IntPtr terrain, NativeStreamOptions options)
+		{
+			HandleResultCode(SvoNative.svo_stream_update(_ctx, terrain, ref options, out NativeStreamStats stats), "StreamUpdate");
+			return stats;
 		}
 
 		public void Dispose()

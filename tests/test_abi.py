@@ -117,3 +117,43 @@ def test_camera_lut_values(svo):
         u = np.float32(cam.fov[0]) * ((np.float32(4.0) - np.float32(x)) / np.float32(8.0))
         assert cu[x] == np.float32(np.cos(np.float64(u))) and su[x] == np.float32(np.sin(np.float64(u)))
     assert cu[4] == 1.0 and su[4] == 0.0
+
+
+def test_struct_layouts_match_the_c_compiler(svo, tmp_path):
+    """Sizes and field offsets of the ABI structs as gcc lays them out from include/svo.h,
+    against the Python binding's ctypes mirrors."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    checks = {
+        "svo_trace_input": (svo.TraceInput, ["tick", "max_child_request_id", "ambient_light_level"]),
+        "svo_update_input_data": (svo.UpdateInputData, ["tick", "offset", "graft_size"]),
+        "svo_frame_options": (svo.FrameOptions, ["stack_levels", "work_chunk", "waves_per_simd"]),
+        "svo_stream_options": (svo.StreamOptions, ["tick", "max_bricks", "prune_age", "gc_free_percent", "flags"]),
+        "svo_stream_stats": (svo.StreamStats, ["unique", "moved", "free_groups", "pool_groups"]),
+        "svo_composite_options": (svo.CompositeOptions, ["max_layers", "flags"]),
+        "svo_adam_options": (svo.AdamOptions, ["epsilon", "step"]),
+        "svo_tree_stats": (svo.TreeStats, ["requests", "unlinks"]),
+        "svo_geometry": (svo.Geometry, ["seed", "resolution", "frequency", "z_max"]),
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "svo.h"', "int main(void) {"]
+    for st, (_, fields) in checks.items():
+        lines.append(f'  printf("{st} size %zu\\n", sizeof({st}));')
+        for f in fields:
+            lines.append(f'  printf("{st} {f} %zu\\n", offsetof({st}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        st, key, val = line.split()
+        got[(st, key)] = int(val)
+    for st, (cls, fields) in checks.items():
+        assert got[(st, "size")] == C.sizeof(cls), st
+        for f in fields:
+            assert got[(st, f)] == getattr(cls, f).offset, (st, f)
