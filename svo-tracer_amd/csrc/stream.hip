@@ -135,6 +135,9 @@ __global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_requ
   // never keys: addresses out of range, and requests whose cone needs no level below the
   // block (TreeSize 0, wavefront.cl:362-375); request_flag_kernel rejects them
   if (a >= n_blocks || req[i].tree_size == 0) return;
+  // neighbouring cones mostly ask for the same block: a request right after an inserted
+  // request for the same address is not a first occurrence and needs no atomic
+  if (i > 0 && req[i - 1].address == a && req[i - 1].tree_size != 0) return;
   uint32_t h = addr_hash(a, mask);
   for (uint32_t probe = 0; probe <= mask; probe++) {
     uint32_t k = keys[h];
@@ -156,8 +159,10 @@ __global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_reque
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
   if (i >= n) return;
   const svo_child_request r = req[i];
-  // requests request_first_kernel never inserted are not looked up
-  if (r.address >= n_blocks || r.tree_size == 0) {
+  // requests request_first_kernel never inserted are not looked up, nor requests right after
+  // an inserted request for the same address
+  if (r.address >= n_blocks || r.tree_size == 0 ||
+      (i > 0 && req[i - 1].address == r.address && req[i - 1].tree_size != 0)) {
     flags[i] = 0u;
     return;
   }

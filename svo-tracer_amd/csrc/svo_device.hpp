@@ -19,12 +19,14 @@ constexpr uint64_t kFnvOffset = 0xcbf29ce484222325ull;
 constexpr uint64_t kFnvPrime = 0x100000001b3ull;
 
 // floatToUlong, wavefront.cl:200-207 (saturating, NaN -> 0 where the reference is undefined).
+// Branch-free: the conversion always runs on an in-range value (0 for NaN and overflow) and
+// the saturated cases are selected afterwards, so lanes never split on it.
 __device__ __forceinline__ uint64_t float_to_ulong(float x) {
-  if (x >= 0.999999911f) return kU64Max;
   const float a = fabsf(x) * 18446744073709551616.0f;
-  if (a != a) return 0;
-  if (a >= 18446744073709551616.0f) return kU64Max;
-  return (uint64_t)a;
+  const bool in_range = a < 18446744073709551616.0f;  // false for NaN and >= 2^64
+  const bool saturate = (x >= 0.999999911f) | (a >= 18446744073709551616.0f);
+  const uint64_t v = (uint64_t)(in_range ? a : 0.0f);
+  return saturate ? kU64Max : v;
 }
 
 // ulongToFloat, wavefront.cl:209: exact scaling by 2^-64.

@@ -105,6 +105,20 @@ __device__ ResolveOut resolve_target(const PoolView &p, uint32_t depth, uint32_t
 }
 
 // Frontier append (order-free: the free walk is a set operation).
+// counters[slot] += 1 for every active lane, with one atomic per distinct slot per wave
+// (every lane of a wave otherwise hits the same few counters).
+__device__ __forceinline__ void wave_count(uint32_t *counters, uint32_t slot) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t pending = __ballot(1);
+  while (pending) {
+    const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1u;
+    const uint32_t s = __shfl(slot, (int)leader);
+    const uint64_t same = __ballot(slot == s);
+    if (lane == leader) atomicAdd(&counters[s], (uint32_t)__popcll(same));
+    pending &= ~same;
+  }
+}
+
 __device__ __forceinline__ void push(uint2 *front, uint32_t *count, uint32_t cap, uint32_t *overflow, uint32_t g,
                                      uint32_t pb) {
   const uint32_t i = atomicAdd(count, 1u);
@@ -186,7 +200,7 @@ __global__ __launch_bounds__(kFlat) void prune_resolve_kernel(PoolView p, const 
   const uint32_t slot = (o.status == kResOk || o.status == kResBase) ? kTcApplied
                         : o.status == kResInvalid                   ? kTcInvalid
                                                                     : kTcSkipped;
-  atomicAdd(&counters[slot], 1u);
+  wave_count(counters, slot);
 }
 
 __global__ __launch_bounds__(kFlat) void graft_resolve_kernel(PoolView p, const svo_grafting *rec, uint32_t n,
@@ -602,9 +616,9 @@ __global__ __launch_bounds__(kFlat) void graft_roots_kernel(PoolView p, ResolveO
                       win_get(keys, vals, mask, win_key(kClsGraft, o.addr)) == i;
   res[i].placed = placed ? 1u : 0u;
   const uint32_t slot = placed ? kTcPlaced : o.status == kResInvalid ? kTcInvalid : kTcSkipped;
-  atomicAdd(&counters[slot], 1u);
-  if (placed) atomicAdd(&counters[kTcApplied], 1u);
+  wave_count(counters, slot);
   if (!placed) return;
+  wave_count(counters, kTcApplied);
   const uint32_t c = p.blocks[o.addr].child;
   if (c != kU32Max) push(front, &counters[kTcFront0], cap, &counters[kTcOverflow], c >> 3, o.addr);
 }
