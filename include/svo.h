@@ -271,7 +271,13 @@ int svo_set_arg_buffer(svo_context *ctx, int kernel_name, const char *param, int
 /* SetArg<T>(KernelName, string, T / T[]), ComputeManager.cs:80-92 (by-value structs such as
  * TraceInput / UpdateInputData, size checked against the natural layout). */
 int svo_set_arg_bytes(svo_context *ctx, int kernel_name, const char *param, const void *value, size_t bytes);
-/* Enqueue(KernelName, nuint[] workSize), ComputeManager.cs:72-78: global size only. */
+/* Enqueue(KernelName, nuint[] workSize), ComputeManager.cs:72-78: global size only.  Queue
+ * appends are in work-item order.  EvaluateMaterial (wavefront.cl:1568-1739) hands out
+ * accumulator ids from AccumulatorID in canonical order (material entries in queue order,
+ * each entry's cones in the order of the reference's loops) and appends the cones with
+ * weighting > 5 to BackgroundQueue in id order; sin/cos/normalize are defined functions
+ * (Cody-Waite reduction + Taylor polynomials; v / sqrt(v.v)), cones beyond the trace arrays
+ * are counted but not stored, and the spine/ring loops stop after 256/4096 steps. */
 int svo_enqueue(svo_context *ctx, int kernel_name, uint32_t dims, const size_t *global_size);
 /* Flush, ComputeManager.cs:277-281. */
 int svo_flush(svo_context *ctx);

@@ -47,6 +47,24 @@ class Pool(C.Structure):
     ]
 
 
+class MaterialIO(C.Structure):
+    _fields_ = [
+        ("queue", C.c_void_p), ("n", C.c_uint32),
+        ("root_directions", C.c_void_p), ("root_locations", C.c_void_p),
+        ("root_depths", C.c_void_p), ("root_weightings", C.c_void_p), ("root_parents", C.c_void_p),
+        ("n_root", C.c_uint32),
+        ("blocks", C.c_void_p), ("n_blocks", C.c_uint32),
+        ("usages", C.c_void_p), ("n_usages", C.c_uint32),
+        ("acc_counter", C.c_void_p),
+        ("origins", C.c_void_p), ("directions", C.c_void_p), ("fovs", C.c_void_p), ("locations", C.c_void_p),
+        ("depths", C.c_void_p), ("weightings", C.c_void_p), ("colour_r", C.c_void_p), ("colour_g", C.c_void_p),
+        ("colour_b", C.c_void_p), ("luminosities", C.c_void_p), ("parents", C.c_void_p),
+        ("n_out", C.c_uint32),
+        ("final_w", C.c_void_p), ("n_final", C.c_uint32),
+        ("bg_queue", C.c_void_p), ("bg_counter", C.c_void_p), ("bg_capacity", C.c_uint32),
+    ]
+
+
 class TreeStats(C.Structure):
     _fields_ = [(k, C.c_uint32) for k in ("applied", "skipped", "invalid", "requests", "collected", "placed", "unlinks")]
 
@@ -112,6 +130,10 @@ def lib() -> C.CDLL:
         L.orc_reclaim.restype = C.c_uint32
         L.orc_compact.argtypes = [C.POINTER(Pool), C.c_void_p]
         L.orc_compact.restype = C.c_uint32
+        L.orc_evaluate_material.argtypes = [C.c_void_p, C.POINTER(MaterialIO)]
+        L.orc_evaluate_material.restype = C.c_int
+        L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_sincos.restype = None
         L.orc_tick_age.argtypes = [C.c_uint16, C.c_uint16]
         L.orc_tick_age.restype = C.c_uint32
         _lib = L
@@ -483,3 +505,53 @@ def composite_loss(layers: np.ndarray, bg: np.ndarray, params: np.ndarray, targe
                                     None if t is None else t.ctypes.data, None if image is None else image.ctypes.data,
                                     None if grads is None else grads.ctypes.data)
     return float(loss), image, grads
+
+
+# ---------------------------------------------------------------------------
+# evaluateMaterial (secondary cones)
+# ---------------------------------------------------------------------------
+
+def sincos(x: float):
+    """The defined sin and cos of evaluateMaterial (see svo_material_oracle.h)."""
+    s, c = C.c_float(0.0), C.c_float(0.0)
+    lib().orc_sincos(C.c_float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+MATERIAL_OUTPUTS = ("origins", "directions", "fovs", "locations", "depths", "weightings", "colour_r", "colour_g",
+                    "colour_b", "luminosities", "parents")
+
+
+def evaluate_material(trace_input, queue: np.ndarray, root: dict, blocks: np.ndarray, usages: Optional[np.ndarray],
+                      out: dict, final_w: np.ndarray, bg_queue: np.ndarray, counters: np.ndarray) -> None:
+    """evaluateMaterial over `queue` with the primary state in `root` (directions float4,
+    locations u64x4, depths, weightings, parents) into the accumulator arrays of `out`
+    (origins/directions float4, fovs, locations u64x4, depths, weightings, colour_r/g/b,
+    luminosities, parents), FinalWeightings and the background queue; counters =
+    [AccumulatorID, BackgroundQueueID] as uint32, advanced in place.  All arrays in place."""
+    io = MaterialIO()
+    q = np.ascontiguousarray(queue, np.uint32)
+    io.queue, io.n = q.ctypes.data, q.size
+    io.root_directions = root["directions"].ctypes.data
+    io.root_locations = root["locations"].ctypes.data
+    io.root_depths = root["depths"].ctypes.data
+    io.root_weightings = root["weightings"].ctypes.data
+    io.root_parents = root["parents"].ctypes.data
+    io.n_root = min(root["directions"].size // 4, root["locations"].size // 4, root["depths"].size,
+                    root["weightings"].size, root["parents"].size)
+    io.blocks, io.n_blocks = blocks.ctypes.data, blocks.size
+    if usages is not None:
+        io.usages, io.n_usages = usages.ctypes.data, usages.size
+    io.acc_counter = counters.ctypes.data
+    io.bg_counter = counters.ctypes.data + 4
+    for k in MATERIAL_OUTPUTS:
+        setattr(io, k, out[k].ctypes.data)
+    io.n_out = min(out["origins"].size // 4, out["directions"].size // 4, out["fovs"].size, out["locations"].size // 4,
+                   *(out[k].size for k in ("depths", "weightings", "colour_r", "colour_g", "colour_b", "luminosities",
+                                          "parents")))
+    io.final_w, io.n_final = final_w.ctypes.data, final_w.size
+    io.bg_queue, io.bg_capacity = bg_queue.ctypes.data, bg_queue.size
+    assert counters.dtype == np.uint32 and counters.size == 2 and counters.flags["C_CONTIGUOUS"]
+    rc = lib().orc_evaluate_material(C.byref(trace_input), C.byref(io))
+    if rc != 0:
+        raise RuntimeError(f"orc_evaluate_material failed: {rc}")

@@ -218,6 +218,8 @@ __device__ __forceinline__ uint32_t block_chunk(uint4 q) { return q.y & 0xffffu;
 __device__ __forceinline__ uint32_t block_r(uint4 q) { return (q.z >> 16) & 0xffu; }
 __device__ __forceinline__ uint32_t block_g(uint4 q) { return q.z >> 24; }
 __device__ __forceinline__ uint32_t block_b(uint4 q) { return q.w & 0xffu; }
+__device__ __forceinline__ int16_t block_pitch(uint4 q) { return (int16_t)(q.y >> 16); }
+__device__ __forceinline__ int16_t block_yaw(uint4 q) { return (int16_t)(q.z & 0xffffu); }
 
 }  // namespace dev
 }  // namespace svo

@@ -104,6 +104,8 @@ struct Context {
   uint32_t *tree_host = nullptr;
   // scratch of streaming (request consumption, compaction) and its pinned counter mirror
   std::shared_ptr<DeviceBuffer> stream_scratch[12];
+  // scratch of evaluateMaterial: per-entry counts and first ids, chunk sums, queue flags
+  std::shared_ptr<DeviceBuffer> material_scratch[5];
   uint32_t *stream_host = nullptr;
   // batched cameras: concatenated camera tables (cache key), camera records
   std::shared_ptr<DeviceBuffer> scratch_batch_lut;

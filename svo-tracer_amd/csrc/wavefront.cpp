@@ -10,6 +10,7 @@
 #include "svo_internal.hpp"
 #include "svo_runtime.hpp"
 #include "svo_wavefront.hpp"
+#include "svo_material_kernels.hpp"
 #include "svo_wavefront_kernels.hpp"
 
 namespace svo {
@@ -296,6 +297,88 @@ int enqueue_draw(Context *ctx, Resolver &r, uint32_t gsx, uint32_t gsy) {
   return SVO_OK;
 }
 
+// evaluateMaterial, wavefront.cl:1568-1739 (MainWindow.EvaulateMaterial binds the Root*
+// buffers after swapping them with the current trace arrays, MainWindow.cs:533-571).
+int enqueue_material(Context *ctx, Resolver &r, uint32_t n) {
+  svo_trace_input in;
+  if (!r.input(&in)) return r.rc;
+  auto blocks = r.buf("Blocks"), usages = r.buf("Usages", false), queue = r.buf("MaterialQueue");
+  auto rd = r.buf("RootDirections"), rl = r.buf("RootLocations"), rdp = r.buf("RootDepths"),
+       rw = r.buf("RootWeightings"), rp = r.buf("RootParentTraces");
+  auto o = r.buf("Origins"), d = r.buf("Directions"), f = r.buf("FoVs"), l = r.buf("Locations"), dp = r.buf("Depths"),
+       w = r.buf("Weightings"), cr = r.buf("ColourRs"), cg = r.buf("ColourGs"), cb = r.buf("ColourBs"),
+       lum = r.buf("Luminosities"), pt = r.buf("ParentTraces"), fw = r.buf("FinalWeightings");
+  auto acc = r.buf("AccumulatorID"), bgq = r.buf("BackgroundQueue"), bgc = r.buf("BackgroundQueueID");
+  if (r.rc) return r.rc;
+  int rc;
+  if ((rc = check_counter(r, acc, "AccumulatorID")) || (rc = check_counter(r, bgc, "BackgroundQueueID"))) return rc;
+  if (queue.bytes < (size_t)n * 4) return fail(SVO_ERR_BUFFER_TOO_SMALL, "evaluateMaterial: work size exceeds MaterialQueue");
+  // the reference kernel returns at once for these parameters
+  if (in.fov_constant <= 0.0f || in.fov_multiplier >= 1.0f || in.fov_multiplier <= -1.0f) return SVO_OK;
+  MaterialParams p;
+  std::memset(&p, 0, sizeof p);
+  p.queue = (const uint32_t *)queue.ptr;
+  p.n = n;
+  p.root_directions = (const float4 *)rd.ptr;
+  p.root_locations = (const uint64_t *)rl.ptr;
+  p.root_depths = (const uint8_t *)rdp.ptr;
+  p.root_weightings = (const uint8_t *)rw.ptr;
+  p.root_parents = (const uint32_t *)rp.ptr;
+  p.n_root = std::min({count_of(rd, 16), count_of(rl, 32), count_of(rdp, 1), count_of(rw, 1), count_of(rp, 4)});
+  p.blocks = (const uint4 *)blocks.ptr;
+  p.n_blocks = count_of(blocks, 16);
+  p.usage16 = (uint16_t *)usages.ptr;
+  p.n_usages = usages.ptr ? count_of(usages, 8) : 0;
+  p.B = in.base_depth;
+  p.tick = in.tick;
+  p.fov_multiplier = in.fov_multiplier;
+  p.fov_constant = in.fov_constant;
+  p.weighting_multiplier = in.weighting_multiplier;
+  p.weighting_constant = in.weighting_constant;
+  p.acc_counter = (uint32_t *)acc.ptr;
+  p.origins = (float4 *)o.ptr;
+  p.directions = (float4 *)d.ptr;
+  p.fovs = (float *)f.ptr;
+  p.locations = (uint64_t *)l.ptr;
+  p.depths = (uint8_t *)dp.ptr;
+  p.weightings = (uint8_t *)w.ptr;
+  p.colour_r = (uint8_t *)cr.ptr;
+  p.colour_g = (uint8_t *)cg.ptr;
+  p.colour_b = (uint8_t *)cb.ptr;
+  p.luminosities = (uint8_t *)lum.ptr;
+  p.parents = (uint32_t *)pt.ptr;
+  p.n_out = std::min({count_of(o, 16), count_of(d, 16), count_of(f, 4), count_of(l, 32), count_of(dp, 1),
+                      count_of(w, 1), count_of(cr, 1), count_of(cg, 1), count_of(cb, 1), count_of(lum, 1),
+                      count_of(pt, 4)});
+  p.final_w = (uint32_t *)fw.ptr;
+  p.n_final = count_of(fw, 4);
+  hipStream_t s = ctx->stream;
+  auto &sc = ctx->material_scratch;
+  const uint32_t chunks = material_chunks(n);
+  if ((rc = ensure_scratch(sc[0], (size_t)n * 4)) || (rc = ensure_scratch(sc[1], (size_t)n * 4)) ||
+      (rc = ensure_scratch(sc[2], ((size_t)chunks + 1) * 4)))
+    return rc;
+  uint32_t *counts = (uint32_t *)sc[0]->ptr, *first = (uint32_t *)sc[1]->ptr, *sums = (uint32_t *)sc[2]->ptr;
+  if (launch_material_count(p, counts, s) || launch_material_scan(counts, n, first, sums, s))
+    return hip_fail(hipGetLastError(), "evaluateMaterial: count");
+  uint32_t total = 0;
+  hipError_t e = hipMemcpyAsync(&total, sums + chunks, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "evaluateMaterial: total");
+  const uint32_t fchunks = material_chunks(total);
+  if ((rc = ensure_scratch(sc[3], (size_t)std::max(total, 1u) * 4)) || (rc = ensure_scratch(sc[4], ((size_t)fchunks + 1) * 4)))
+    return rc;
+  p.bg_flags = (uint32_t *)sc[3]->ptr;
+  p.flag_capacity = total;
+  uint32_t *fsums = (uint32_t *)sc[4]->ptr;
+  if (launch_material_write(p, first, s) || launch_material_scan(p.bg_flags, total, nullptr, fsums, s) ||
+      launch_material_append(p.bg_flags, total, fsums, p.acc_counter, (const uint32_t *)bgc.ptr, (uint32_t *)bgq.ptr,
+                             count_of(bgq, 4), s) ||
+      launch_material_counters(p.acc_counter, (uint32_t *)bgc.ptr, sums + chunks, fsums + fchunks, s))
+    return hip_fail(hipGetLastError(), "evaluateMaterial launch");
+  return SVO_OK;
+}
+
 // KernelName.Trace: the single-kernel form (kernel.cl:1531-1558) maps onto the fused frame.
 int enqueue_trace(Context *ctx, Resolver &r, uint32_t gsx, uint32_t gsy) {
   svo_trace_input in;
@@ -335,6 +418,8 @@ int enqueue_wavefront(Context *ctx, int kernel, uint32_t dims, const size_t *g) 
       return enqueue_shade(ctx, r, gsx * gsy, false);
     case SVO_KERNEL_DRAW_TRACE:
       return enqueue_draw(ctx, r, gsx, gsy);
+    case SVO_KERNEL_EVALUATE_MATERIAL:
+      return enqueue_material(ctx, r, gsx * gsy);
     case SVO_KERNEL_TRACE:
       return enqueue_trace(ctx, r, gsx, gsy);
     case SVO_KERNEL_PRUNE:

@@ -14,8 +14,11 @@ from frame_helpers import first_mismatch, tree_arrays
 pytestmark = pytest.mark.gpu
 
 
-def setup_buffers(svo, cm, bases, blocks, usages, n, request_capacity):
+def setup_buffers(svo, cm, bases, blocks, usages, n, request_capacity, trace_capacity=None):
+    """Buffers of MainWindow (MainWindow.cs:121-178); per-trace arrays hold trace_capacity
+    entries (default n, one per pixel)."""
     B = svo.BufferName
+    m = trace_capacity or n
     cm.InitReadBuffer(B.Bases, bases)
     cm.InitReadBuffer(B.Blocks, blocks)
     cm.InitBuffer(B.Usages, usages)
@@ -23,9 +26,11 @@ def setup_buffers(svo, cm, bases, blocks, usages, n, request_capacity):
     cm.InitBuffer(B.ChildRequests, np.zeros(max(request_capacity, 1) * 40, np.uint8))
     for name, size in [(B.Origins, 16), (B.Directions, 16), (B.FoVs, 4), (B.Locations, 32), (B.Depths, 1),
                        (B.Weightings, 1), (B.ParentTraces, 4), (B.ColourRs, 1), (B.ColourGs, 1), (B.ColourBs, 1),
-                       (B.FinalColourRs, 4), (B.FinalColourGs, 4), (B.FinalColourBs, 4), (B.FinalWeightings, 4),
                        (B.BaseTraces, 4), (B.BaseTraceQueue, 4), (B.BlockTraces, 4), (B.BlockTraceQueue, 4),
-                       (B.BackgroundQueue, 4), (B.MaterialQueue, 4), (B.OutputImage, 16)]:
+                       (B.BackgroundQueue, 4), (B.MaterialQueue, 4)]:
+        cm.InitDeviceBuffer(name, m, size)
+    for name, size in [(B.FinalColourRs, 4), (B.FinalColourGs, 4), (B.FinalColourBs, 4), (B.FinalWeightings, 4),
+                       (B.OutputImage, 16)]:
         cm.InitDeviceBuffer(name, n, size)
     for name in (B.BaseTraceQueueID, B.BlockTraceQueueID, B.BackgroundQueueID, B.MaterialQueueID, B.AccumulatorID):
         cm.InitBuffer(name, np.zeros(1, np.uint32))
@@ -38,6 +43,13 @@ def bind(svo, cm, kernel, names):
 
 def run_wavefront(svo, cm, trace_input):
     """MainWindow.RunWavefront + RunTraces + per-kernel binders, through the ComputeManager mirror."""
+    iterations, bg = run_primary(svo, cm, trace_input)
+    mat = finish_frame(svo, cm, trace_input)
+    return iterations, bg, mat
+
+
+def run_primary(svo, cm, trace_input):
+    """init, the RunTraces loop and evaluateBackground of MainWindow.RunWavefront."""
     K, B = svo.KernelName, svo.BufferName
     W, H = trace_input.screen_size[0], trace_input.screen_size[1]
     for name in (B.BlockTraceQueueID, B.BaseTraceQueueID, B.BackgroundQueueID, B.MaterialQueueID,
@@ -86,6 +98,13 @@ def run_wavefront(svo, cm, trace_input):
                                              "ColourGs", "ColourBs", "Weightings", "FinalColourRs",
                                              "FinalColourGs", "FinalColourBs", "FinalWeightings"])
         cm.Enqueue(K.EvaluateBackground, [bg])
+    return iterations, bg
+
+
+def finish_frame(svo, cm, trace_input):
+    """resolveRemainders and drawTrace of MainWindow.RunWavefront; returns the material count."""
+    K, B = svo.KernelName, svo.BufferName
+    W, H = trace_input.screen_size[0], trace_input.screen_size[1]
     cm.SetArg(K.ResolveRemainders, "input", trace_input)
     bind(svo, cm, K.ResolveRemainders, ["Blocks", "Usages", "Locations", "Depths", "MaterialQueue",
                                         "FinalColourRs", "FinalColourGs", "FinalColourBs", "FinalWeightings",
@@ -98,7 +117,7 @@ def run_wavefront(svo, cm, trace_input):
     cm.SetArg(K.DrawTrace, "input", trace_input)
     cm.Enqueue(K.DrawTrace, [W, H])
     cm.Wait()
-    return iterations, bg, mat
+    return mat
 
 
 CASES = {
