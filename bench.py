@@ -63,13 +63,39 @@ def cpu_baseline(svo, tree, cams, seconds: float):
         oracle.trace_rays(cams[frames], bases, blocks, ids, threads=threads)
         spent += time.perf_counter() - t0
         frames += 1
+    model = "unknown CPU"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": round(frames * n / spent / 1e6, 4),
         "unit": "Mcones/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{frames} whole frames of the same orbit ({frames * n} cones, {spent:.1f} s host time)",
+        "sample": f"{frames} whole frames of the same orbit ({frames * n} cones, {spent:.1f} s host time) "
+                  f"on {threads} threads of {model} ({os.cpu_count()} logical CPUs visible)",
     }
+
+
+def copy_bandwidth(svo, mib: int = 2048, reps: int = 5) -> float:
+    """GB/s of a device-to-device copy of `mib` MiB (read + write bytes over time): the
+    practical HBM ceiling on this GPU, reported beside the 8 TB/s peak."""
+    B = svo.BufferName
+    n = mib << 20
+    with svo.ComputeManager(0) as cm:
+        cm.InitDeviceBuffer(B.RootLocations, n)
+        cm.InitDeviceBuffer(B.RootDirections, n)
+        cm.CopyBuffer(B.RootDirections, B.RootLocations, n)
+        cm.Wait()
+        cm.TimerBegin()
+        for _ in range(reps):
+            cm.CopyBuffer(B.RootDirections, B.RootLocations, n)
+        ms = cm.TimerEnd()
+    return 2.0 * n * reps / (ms / 1e3) / 1e9
 
 
 def graph_opts(opts: dict) -> dict:
@@ -440,6 +466,11 @@ def main() -> int:
             "traffic": traffic,
         },
     }
+    copy_gbs = copy_bandwidth(svo)
+    line["hbm_copy"] = {"value": round(copy_gbs, 1), "unit": "GB/s",
+                        "what": "2 GiB device-to-device copy, read + write bytes",
+                        "achieved_vs_copy": round(achieved / copy_gbs, 4)}
+    log(f"device copy: {copy_gbs:.0f} GB/s; trace kernel at {achieved / copy_gbs:.3f} of it")
     if not args.no_batched:
         log("batched 64-camera leg...")
         line["batched"] = batched_leg(svo, cm, max(3, min(K, 10)), 2, usages0, opts)
