@@ -145,6 +145,86 @@ __device__ __forceinline__ void traverse_chunk(uint32_t depth, uint64_t &x, uint
   z = advance_axis(z, n2, sz);
 }
 
+// ---- sign-canonical positions (the fused trace kernel) ----
+// A cone's position is kept per axis as c = x for an axis it moves along towards 0 and
+// c = ~x (= ULONG_MAX - x) for an axis it moves along towards ULONG_MAX, so that on every
+// axis the cone moves towards c = 0.  The map is an involution (x = c ^ mask) and turns
+// each sign-dependent rule of the step into one rule:
+//   roundUlong up from x = (~x & lowmask) + 1 = (c & lowmask) + 1; roundUlong down from x
+//   is the same except in the first cell (x <= lowmask), where it is x itself;
+//   x + n (up) is ~(c - n) and x - n (down) is c - n, and landing on the wrapped-around
+//   face (x = 0 up, x = ULONG_MAX down) is c - n = ULONG_MAX in both cases;
+//   leaving (x on the face it moves towards) is c = 0;
+//   comparePositions' "bit held the far value and lost it" is (pc ^ c) & ~pc;
+//   chunkPosition's bit of an up axis is the complement of c's bit.
+__device__ __forceinline__ uint64_t sign_mask64(uint32_t signs, uint32_t axis) {
+  return (uint64_t)0 - (uint64_t)((signs >> axis) & 1u);
+}
+__device__ __forceinline__ uint32_t sign_mask32(uint32_t signs, uint32_t axis) { return 0u - ((signs >> axis) & 1u); }
+
+__device__ __forceinline__ uint64_t round_canon(uint64_t c, uint32_t depth, bool pos) {
+  const uint64_t lowmask = kU64Max >> (depth + 1);
+  return (!pos && c <= lowmask) ? c : (c & lowmask) + 1;
+}
+
+__device__ __forceinline__ uint64_t advance_canon(uint64_t c, uint64_t n) {
+  const uint64_t v = c - n;
+  return v == kU64Max ? 0ull : v;
+}
+
+// traverse_chunk on sign-canonical positions (same floating-point operations in the same
+// order; only the integer bookkeeping of the signs is gone).
+__device__ __forceinline__ void traverse_canon(uint32_t depth, uint64_t &cx, uint64_t &cy, uint64_t &cz, float dx,
+                                               float dy, float dz, float ix, float iy, float iz, uint32_t signs) {
+  const uint64_t d0 = round_canon(cx, depth, signs & 1u), d1 = round_canon(cy, depth, signs & 2u),
+                 d2 = round_canon(cz, depth, signs & 4u);
+  const float f0 = (float)d0, f1 = (float)d1, f2 = (float)d2;
+  const float a0 = fabsf(f0 * ix), a1 = fabsf(f1 * iy), a2 = fabsf(f2 * iz);
+  const int m = (a0 <= a1 && a0 <= a2) ? 0 : ((a1 <= a0 && a1 <= a2) ? 1 : 2);
+  const float inv_m = m == 0 ? ix : (m == 1 ? iy : iz);
+  const float ud = (m == 0 ? f0 : (m == 1 ? f1 : f2)) * 5.42101086242752217e-20f;
+  const uint64_t dm = m == 0 ? d0 : (m == 1 ? d1 : d2);
+  const uint64_t na = float_to_ulong(((m == 0 ? dy : dx) * inv_m) * ud);
+  const uint64_t nb = float_to_ulong(((m == 2 ? dy : dz) * inv_m) * ud);
+  const uint64_t n0 = m == 0 ? dm : na;
+  const uint64_t n1 = m == 1 ? dm : (m == 0 ? na : nb);
+  const uint64_t n2 = m == 2 ? dm : nb;
+  cx = advance_canon(cx, n0);
+  cy = advance_canon(cy, n1);
+  cz = advance_canon(cz, n2);
+}
+
+__device__ __forceinline__ bool leaving_canon(uint64_t cx, uint64_t cy, uint64_t cz) {
+  return cx == 0ull || cy == 0ull || cz == 0ull;
+}
+
+// compare_positions on sign-canonical positions.
+__device__ __forceinline__ uint32_t compare_canon(uint32_t depth, uint64_t px, uint64_t py, uint64_t pz, uint64_t x,
+                                                  uint64_t y, uint64_t z) {
+  uint32_t k;
+  if (depth <= 31u) {
+    const uint32_t pxh = (uint32_t)(px >> 32), pyh = (uint32_t)(py >> 32), pzh = (uint32_t)(pz >> 32);
+    const uint32_t c = ((pxh ^ (uint32_t)(x >> 32)) & ~pxh) | ((pyh ^ (uint32_t)(y >> 32)) & ~pyh) |
+                       ((pzh ^ (uint32_t)(z >> 32)) & ~pzh);
+    const uint32_t t = ~(c >> (31u - depth));
+    k = t ? (uint32_t)__builtin_ctz(t) : 32u;
+  } else {
+    const uint64_t c = ((px ^ x) & ~px) | ((py ^ y) & ~py) | ((pz ^ z) & ~pz);
+    k = (uint32_t)__builtin_ctzll(~(c >> (63u - depth)));
+  }
+  return depth > k ? depth - k : 1u;
+}
+
+// Morton code of the top `depth` bits (depth <= 10) from the high words of the positions.
+__device__ __forceinline__ uint32_t morton_top_hi(uint32_t depth, uint32_t xh, uint32_t yh, uint32_t zh) {
+  const uint32_t s = 32u - depth;
+  return spread3(xh >> s) | (spread3(yh >> s) << 1) | (spread3(zh >> s) << 2);
+}
+__device__ __forceinline__ uint32_t morton_top_base_hi(uint32_t depth, uint32_t xh, uint32_t yh, uint32_t zh) {
+  const uint32_t s = 32u - depth;
+  return spread3_7(xh >> s) | (spread3_7(yh >> s) << 1) | (spread3_7(zh >> s) << 2);
+}
+
 // coneSize(fast_length(Origin - loc)), wavefront.cl:387-399, returning the cone itself.
 __device__ __forceinline__ float cone_at(float ox, float oy, float oz, uint64_t x, uint64_t y, uint64_t z, float fov,
                                          float dof0, float dof1) {

@@ -78,7 +78,9 @@ trace_frame_kernel(const FrameParams p) {
 
   bool active = false;
   uint32_t ray = 0, depth = 0, signs = 0, node_addr = 0, last_group = kU32Max, nvis = 0, nblk = 0;
-  uint64_t lx = 0, ly = 0, lz = 0, hash = kFnvOffset;
+  // the location in sign-canonical form (c = x on an axis the cone moves down, ~x on one it
+  // moves up: every axis then moves towards 0); loc(a) gives the location itself
+  uint64_t cx = 0, cy = 0, cz = 0, hash = kFnvOffset;
   float dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
   uint4 node = make_uint4(0u, 0u, 0u, 0u);
   uint32_t tot_vis = 0, tot_blk = 0;
@@ -101,7 +103,7 @@ trace_frame_kernel(const FrameParams p) {
     // deeper than the stack: walk down from its last level along the current location
     uint32_t a = s_stack[(S - 1u) * WG + tid];
     for (uint32_t L = B + 2u + S - 1u; L < level; L++)
-      a = block_child(load_block(p.blocks, p.n_blocks, a)) + chunk_position(L, lx, ly, lz);
+      a = block_child(load_block(p.blocks, p.n_blocks, a)) + (chunk_position(L, cx, cy, cz) ^ signs);
     return a;
   };
 
@@ -137,9 +139,9 @@ trace_frame_kernel(const FrameParams p) {
     if (p.status) p.status[ray] = (uint8_t)((material ? 1u : 0u) | (child ? 2u : 0u) | (capped ? 4u : 0u));
     if (p.depth_out) p.depth_out[ray] = (uint8_t)depth;
     if (p.loc_out) {
-      p.loc_out[4 * (size_t)ray + 0] = lx;
-      p.loc_out[4 * (size_t)ray + 1] = ly;
-      p.loc_out[4 * (size_t)ray + 2] = lz;
+      p.loc_out[4 * (size_t)ray + 0] = cx ^ sign_mask64(signs, 0u);
+      p.loc_out[4 * (size_t)ray + 1] = cy ^ sign_mask64(signs, 1u);
+      p.loc_out[4 * (size_t)ray + 2] = cz ^ sign_mask64(signs, 2u);
     }
     if (child && p.req_sparse) {
       svo_child_request rq;
@@ -147,9 +149,9 @@ trace_frame_kernel(const FrameParams p) {
       rq.tick = (uint16_t)tick;
       rq.depth = (uint8_t)depth;
       rq.pad0 = 0;
-      rq.location[0] = lx;
-      rq.location[1] = ly;
-      rq.location[2] = lz;
+      rq.location[0] = cx ^ sign_mask64(signs, 0u);
+      rq.location[1] = cy ^ sign_mask64(signs, 1u);
+      rq.location[2] = cz ^ sign_mask64(signs, 2u);
       rq.tree_size = tree_size ? 1 : 0;
       for (int k = 0; k < 7; k++) rq.pad1[k] = 0;
       p.req_sparse[ray] = rq;
@@ -320,12 +322,17 @@ trace_frame_kernel(const FrameParams p) {
             }
             const float o[3] = {camf(cam, 9), camf(cam, 10), camf(cam, 11)};
             const float d[3] = {dx, dy, dz}, inv[3] = {ix, iy, iz};
-            if (trace_into_volume(o, d, inv, camf(cam, 12), lx, ly, lz)) {
+            uint64_t ex, ey, ez;
+            const bool entered = trace_into_volume(o, d, inv, camf(cam, 12), ex, ey, ez);
+            if (!entered) ex = ey = ez = 0;
+            cx = ex ^ sign_mask64(signs, 0u);
+            cy = ey ^ sign_mask64(signs, 1u);
+            cz = ez ^ sign_mask64(signs, 2u);
+            if (entered) {
               depth = 1;
               active = true;
             } else {
               depth = 0;
-              lx = ly = lz = 0;
               if (MODE == kModePlain)
                 retire(false, kU32Max, false, false, false);
               else
@@ -355,7 +362,9 @@ trace_frame_kernel(const FrameParams p) {
     uint32_t index, chunk;
     if (!is_block) {
       // runBaseTrace, wavefront.cl:1400-1463
-      index = base_offset(d) + (B <= 7u ? morton_top_base(d, lx, ly, lz) : morton_top(d, lx, ly, lz));
+      const uint32_t xh = (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u), yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
+                     zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
+      index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
       chunk = p.bases_in_lds ? (uint32_t)s_bases[index] : (uint32_t)p.bases[index];
     } else {
       // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
@@ -371,7 +380,7 @@ trace_frame_kernel(const FrameParams p) {
         }
       }
     }
-    const uint32_t oct = chunk_position(d, lx, ly, lz);
+    const uint32_t oct = chunk_position(d, cx, cy, cz) ^ signs;
     if (HASH) hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
     nvis++;
     nblk += is_block ? 1u : 0u;
@@ -385,13 +394,16 @@ trace_frame_kernel(const FrameParams p) {
           depth = d + 1u;
         } else {
           depth = B + 2u;
-          node_addr = morton_top(B + 2u, lx, ly, lz);
+          node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
+                                    (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
+                                    (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
           stack_put(B + 2u, node_addr);
           node = load_block(p.blocks, p.n_blocks, node_addr);
         }
       } else {
         // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
-        const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), lx, ly, lz, camf(cam, 12),
+        const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
+                                   cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
                                    camf(cam, 13), camf(cam, 14));
         bool hit = false, child = false, tree_size = false;
         if (cone > exp2_neg(B + 2u)) {
@@ -433,9 +445,9 @@ trace_frame_kernel(const FrameParams p) {
     }
     if (advance) {
       // advance to the next cell boundary, pop levels that were left
-      const uint64_t px = lx, py = ly, pz = lz;
-      traverse_chunk(dd, lx, ly, lz, dx, dy, dz, ix, iy, iz, signs);
-      if (leaving(lx, ly, lz, signs)) {
+      const uint64_t px = cx, py = cy, pz = cz;
+      traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
+      if (leaving_canon(cx, cy, cz)) {
         depth = 0;
         if (MODE == kModePlain)
           retire(false, kU32Max, false, false, false);
@@ -443,7 +455,7 @@ trace_frame_kernel(const FrameParams p) {
           end_composite();
         continue;
       }
-      uint32_t nd = compare_positions(dd, px, py, pz, lx, ly, lz, signs);
+      uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
       if (dd > B) {
         if (nd == B + 1u) {
           nd = B;
