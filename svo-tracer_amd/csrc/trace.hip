@@ -346,125 +346,130 @@ trace_frame_kernel(const FrameParams p) {
       if (!fetching) break;
       continue;
     }
-    if (!active) continue;
-
-    // ---- one step of the reference state machine ----
-    if (nvis >= kStepCap) {
-      depth = 0;
-      if (MODE == kModePlain)
-        retire(false, kU32Max, false, false, true);
-      else
-        end_composite();
-      continue;
-    }
-    const uint32_t d = depth;
-    const bool is_block = d > B;
-    uint32_t index, chunk;
-    if (!is_block) {
-      // runBaseTrace, wavefront.cl:1400-1463
-      const uint32_t xh = (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u), yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
-                     zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
-      index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
-      chunk = p.bases_in_lds ? (uint32_t)s_bases[index] : (uint32_t)p.bases[index];
-    } else {
-      // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
-      index = node_addr;
-      chunk = block_chunk(node);
-      const uint32_t g = index >> 3;
-      if (g != last_group) {
-        last_group = g;
-        resolve_usage();
-        if (g < p.n_usages) {
-          pend_g = g;
-          pend_t = p.usage16[4u * (size_t)g];
-        }
-      }
-    }
-    const uint32_t oct = chunk_position(d, cx, cy, cz) ^ signs;
-    if (HASH) hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
-    nvis++;
-    nblk += is_block ? 1u : 0u;
-
-    // empty octant, or (composite modes) a hit that the cone continues past
-    bool advance = ((chunk >> (2u * oct)) & 2u) == 0u;
-    uint32_t dd = d;  // the depth the advance leaves from
-    if (!advance) {
-      if (!is_block) {
-        if (d < B) {
-          depth = d + 1u;
-        } else {
-          depth = B + 2u;
-          node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
-                                    (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
-                                    (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
-          stack_put(B + 2u, node_addr);
-          node = load_block(p.blocks, p.n_blocks, node_addr);
-        }
+    if (active) {
+      // ---- one step of the reference state machine ----
+      // A step that moves to another block only sets node_addr and need_load; the block is
+      // loaded once, at the end of the step.
+      bool need_load = false;
+      if (nvis >= kStepCap) {
+        depth = 0;
+        if (MODE == kModePlain)
+          retire(false, kU32Max, false, false, true);
+        else
+          end_composite();
       } else {
-        // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
-        const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
-                                   cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
-                                   camf(cam, 13), camf(cam, 14));
-        bool hit = false, child = false, tree_size = false;
-        if (cone > exp2_neg(B + 2u)) {
-          depth = B + 2u;
-          node_addr = stack_get(B + 2u);
-          node = load_block(p.blocks, p.n_blocks, node_addr);
-          hit = true;
-        } else if (cone > exp2_neg(d)) {
-          depth = d - 1u;
-          node_addr = stack_get(d - 1u);
-          node = load_block(p.blocks, p.n_blocks, node_addr);
-        } else if (block_child(node) == kU32Max) {
-          // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
-          hit = child = true;
-          tree_size = cone <= exp2_neg(d + 1u);
-        } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
-          const uint32_t a = block_child(node) + oct;
-          depth = d + 1u;
-          stack_put(d + 1u, a);
-          node_addr = a;
-          node = load_block(p.blocks, p.n_blocks, a);
+        const uint32_t d = depth;
+        const bool is_block = d > B;
+        uint32_t index, chunk;
+        if (!is_block) {
+          // runBaseTrace, wavefront.cl:1400-1463
+          const uint32_t xh = (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
+                         yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
+                         zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
+          index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
+          chunk = p.bases_in_lds ? (uint32_t)s_bases[index] : (uint32_t)p.bases[index];
         } else {
-          hit = true;
-        }
-        if (hit) {
-          if (MODE == kModePlain) {
-            retire(true, node_addr, child, tree_size, false);
-          } else {
-            composite(node_addr);
-            if (nlay >= p.max_layers) {
-              end_composite();
-            } else {
-              advance = true;
-              dd = depth;
+          // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
+          index = node_addr;
+          chunk = block_chunk(node);
+          const uint32_t g = index >> 3;
+          if (g != last_group) {
+            last_group = g;
+            resolve_usage();
+            if (g < p.n_usages) {
+              pend_g = g;
+              pend_t = p.usage16[4u * (size_t)g];
             }
           }
         }
-      }
-    }
-    if (advance) {
-      // advance to the next cell boundary, pop levels that were left
-      const uint64_t px = cx, py = cy, pz = cz;
-      traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
-      if (leaving_canon(cx, cy, cz)) {
-        depth = 0;
-        if (MODE == kModePlain)
-          retire(false, kU32Max, false, false, false);
-        else
-          end_composite();
-        continue;
-      }
-      uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
-      if (dd > B) {
-        if (nd == B + 1u) {
-          nd = B;
-        } else if (nd > B + 1u && nd < dd) {
-          node_addr = stack_get(nd);
-          node = load_block(p.blocks, p.n_blocks, node_addr);
+        const uint32_t oct = chunk_position(d, cx, cy, cz) ^ signs;
+        if (HASH) hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
+        nvis++;
+        nblk += is_block ? 1u : 0u;
+
+        // empty octant, or (composite modes) a hit that the cone continues past
+        bool advance = ((chunk >> (2u * oct)) & 2u) == 0u;
+        uint32_t dd = d;  // the depth the advance leaves from
+        if (!advance) {
+          if (!is_block) {
+            if (d < B) {
+              depth = d + 1u;
+            } else {
+              depth = B + 2u;
+              node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
+                                        (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
+                                        (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
+              stack_put(B + 2u, node_addr);
+              need_load = true;
+            }
+          } else {
+            // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
+            const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
+                                       cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
+                                       camf(cam, 13), camf(cam, 14));
+            bool hit = false, child = false, tree_size = false;
+            if (cone > exp2_neg(B + 2u)) {
+              depth = B + 2u;
+              node_addr = stack_get(B + 2u);
+              need_load = true;
+              hit = true;
+            } else if (cone > exp2_neg(d)) {
+              depth = d - 1u;
+              node_addr = stack_get(d - 1u);
+              need_load = true;
+            } else if (block_child(node) == kU32Max) {
+              // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
+              hit = child = true;
+              tree_size = cone <= exp2_neg(d + 1u);
+            } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
+              const uint32_t a = block_child(node) + oct;
+              depth = d + 1u;
+              stack_put(d + 1u, a);
+              node_addr = a;
+              need_load = true;
+            } else {
+              hit = true;
+            }
+            if (hit) {
+              if (MODE == kModePlain) {
+                retire(true, node_addr, child, tree_size, false);
+              } else {
+                composite(node_addr);
+                if (nlay >= p.max_layers) {
+                  end_composite();
+                } else {
+                  advance = true;
+                  dd = depth;
+                }
+              }
+            }
+          }
+        }
+        if (advance) {
+          // advance to the next cell boundary, pop levels that were left
+          const uint64_t px = cx, py = cy, pz = cz;
+          traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
+          if (leaving_canon(cx, cy, cz)) {
+            depth = 0;
+            if (MODE == kModePlain)
+              retire(false, kU32Max, false, false, false);
+            else
+              end_composite();
+          } else {
+            uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
+            if (dd > B) {
+              if (nd == B + 1u) {
+                nd = B;
+              } else if (nd > B + 1u && nd < dd) {
+                node_addr = stack_get(nd);
+                need_load = true;
+              }
+            }
+            depth = nd;
+          }
         }
       }
-      depth = nd;
+      if (need_load) node = load_block(p.blocks, p.n_blocks, node_addr);
     }
   }
 
