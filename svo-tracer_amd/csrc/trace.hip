@@ -407,29 +407,25 @@ trace_frame_kernel(const FrameParams p) {
             const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
                                        cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
                                        camf(cam, 13), camf(cam, 14));
-            bool hit = false, child = false, tree_size = false;
-            if (cone > exp2_neg(B + 2u)) {
-              depth = B + 2u;
-              node_addr = stack_get(B + 2u);
-              need_load = true;
-              hit = true;
-            } else if (cone > exp2_neg(d)) {
-              depth = d - 1u;
-              node_addr = stack_get(d - 1u);
-              need_load = true;
-            } else if (block_child(node) == kU32Max) {
-              // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
-              hit = child = true;
-              tree_size = cone <= exp2_neg(d + 1u);
-            } else if (cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth) {
-              const uint32_t a = block_child(node) + oct;
-              depth = d + 1u;
-              stack_put(d + 1u, a);
-              node_addr = a;
-              need_load = true;
-            } else {
-              hit = true;
+            // the LOD rule's five outcomes: up to level B+2 and hit, up one level, request a
+            // child (hit), down to the child, hit
+            const bool up_top = cone > exp2_neg(B + 2u);
+            const bool up_one = !up_top && cone > exp2_neg(d);
+            const bool stay = !up_top && !up_one;
+            const bool no_child = block_child(node) == kU32Max;
+            // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
+            const bool child = stay && no_child;
+            const bool tree_size = child && cone <= exp2_neg(d + 1u);
+            const bool down = stay && !no_child && cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth;
+            const bool hit = up_top || (stay && !down);
+            depth = up_top ? B + 2u : (up_one ? d - 1u : (down ? d + 1u : d));
+            if (up_top || up_one) {
+              node_addr = stack_get(depth);
+            } else if (down) {
+              node_addr = block_child(node) + oct;
+              stack_put(depth, node_addr);
             }
+            need_load = up_top || up_one || down;
             if (hit) {
               if (MODE == kModePlain) {
                 retire(true, node_addr, child, tree_size, false);
@@ -449,23 +445,22 @@ trace_frame_kernel(const FrameParams p) {
           // advance to the next cell boundary, pop levels that were left
           const uint64_t px = cx, py = cy, pz = cz;
           traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
-          if (leaving_canon(cx, cy, cz)) {
-            depth = 0;
+          const bool leave = leaving_canon(cx, cy, cz);
+          uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
+          // a block level pops to the level it names, except that level B+1 (no blocks)
+          // means the base level B; a pop to a shallower block level re-reads the stack
+          const bool pop_block = !leave && dd > B && nd > B + 1u && nd < dd;
+          nd = (dd > B && nd == B + 1u) ? B : nd;
+          depth = leave ? 0u : nd;
+          if (pop_block) {
+            node_addr = stack_get(nd);
+            need_load = true;
+          }
+          if (leave) {
             if (MODE == kModePlain)
               retire(false, kU32Max, false, false, false);
             else
               end_composite();
-          } else {
-            uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
-            if (dd > B) {
-              if (nd == B + 1u) {
-                nd = B;
-              } else if (nd > B + 1u && nd < dd) {
-                node_addr = stack_get(nd);
-                need_load = true;
-              }
-            }
-            depth = nd;
           }
         }
       }
