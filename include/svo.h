@@ -302,8 +302,13 @@ enum {
   SVO_FRAME_STATE = 1u << 0,  /* also write final Locations (ulong3, 32-B stride) and Depths */
   SVO_FRAME_QUEUES = 1u << 1, /* also write BackgroundQueue/MaterialQueue (+ their ID counters) */
   SVO_FRAME_NO_XCD_SPLIT = 1u << 2,
-  SVO_FRAME_CAMERA_TABLE = 1u << 3 /* accepted for compatibility: camera parameters always come from a
-                                      device table (one record per camera) */
+  SVO_FRAME_CAMERA_TABLE = 1u << 3, /* accepted for compatibility: camera parameters always come from a
+                                       device table (one record per camera) */
+  SVO_FRAME_RASTER_ORDER = 1u << 4   /* take tiles in raster order.  By default the tiles of every band
+                                        are taken most expensive first, by the cost measured in the
+                                        previous frame of the same tiling (the same cameras repeated, as
+                                        in a graph replay or an optimisation loop, gain most); the
+                                        outputs are the same either way */
 };
 
 /* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block

@@ -134,6 +134,7 @@ FRAME_STATE = 1
 FRAME_QUEUES = 2
 FRAME_NO_XCD_SPLIT = 4
 FRAME_CAMERA_TABLE = 8
+FRAME_RASTER_ORDER = 16
 
 GEOMETRY_SPHERES = 0
 GEOMETRY_CUBES = 1

@@ -98,6 +98,7 @@ extern "C" int svo_composite(svo_context *ctx, const svo_trace_input *views, con
   HIP_TRY(hipEventRecord(ctx->cam_copied, ctx->stream), "hipEventRecord");
   if (launch_frame_prepare(p.counters, p.totals, ctx->stream) != 0)
     return hip_fail(hipGetLastError(), "svo_composite: prepare launch");
+  if (launch_schedule(plan, ctx->stream) != 0) return hip_fail(hipGetLastError(), "svo_composite: tile order launch");
   if (launch_composite(p, plan.shape, adjoint ? kModeAdjoint : kModeComposite, ctx->stream) != 0)
     return hip_fail(hipGetLastError(), "svo_composite: launch");
   ctx->last_rays = N;

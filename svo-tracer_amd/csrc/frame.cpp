@@ -266,6 +266,19 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   // defaults: a wave takes one 8x8 tile per claim and refills only when all its lanes are
   // idle -- spatial coherence of the 64 cones matters more than lane utilisation
   p.chunk = o.work_chunk ? ((std::min(o.work_chunk, 1u << 20) + 63u) / 64u) * 64u : 64u;
+  if (!(o.flags & SVO_FRAME_RASTER_ORDER) && tiles > 0) {
+    const DeviceBuffer *old = sc.tile_cost.get();
+    if ((rc = ensure_scratch(sc.tile_cost, (size_t)tiles * 4))) return rc;
+    if ((rc = ensure_scratch(sc.tile_order, (size_t)tiles * 4))) return rc;
+    // costs measured on another tiling (or a new buffer) start from zero: raster order
+    if (sc.tile_cost.get() != old || std::memcmp(sc.sched_ranges, p.range_begin, sizeof p.range_begin) != 0) {
+      if (launch_zero_u32((uint32_t *)sc.tile_cost->ptr, tiles, ctx->stream) != 0)
+        return hip_fail(hipGetLastError(), "trace frame: schedule reset");
+      std::memcpy(sc.sched_ranges, p.range_begin, sizeof p.range_begin);
+    }
+    p.tile_cost = (uint32_t *)sc.tile_cost->ptr;
+    p.order = (const uint32_t *)sc.tile_order->ptr;
+  }
 
   bool bases_in_lds = false;
   uint32_t levels = 1;
@@ -300,6 +313,11 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   return SVO_OK;
 }
 
+int launch_schedule(const BatchPlan &plan, hipStream_t stream) {
+  if (!plan.p.order) return 0;
+  return launch_tile_order(plan.p.tile_cost, (uint32_t *)plan.p.order, plan.p.range_begin, stream);
+}
+
 // The stream work of one planned frame: camera records, counter reset, trace
 // kernel, compaction.  No host synchronisation and no memset nodes (capturable).
 int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool timing) {
@@ -310,6 +328,7 @@ int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, b
   }
   if (launch_frame_prepare(plan.p.counters, plan.p.totals, ctx->stream) != 0)
     return hip_fail(hipGetLastError(), "trace frame: prepare launch");
+  if (launch_schedule(plan, ctx->stream) != 0) return hip_fail(hipGetLastError(), "trace frame: tile order launch");
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
   if (timing && ctx->kernel_timing) {
     if (ctx->kernel_events_used == ctx->kernel_events.size()) {

@@ -66,6 +66,10 @@ struct FrameParams {
   uint32_t stack_levels;
   uint32_t refill;
   uint32_t chunk;  // work items a wave claims per atomic (multiple of 64)
+  // tile schedule (may be null): position k of a band's work range holds tile order[k]; a
+  // wave stores the loop iterations it spent on a tile in tile_cost[tile]
+  const uint32_t *order;
+  uint32_t *tile_cost;
 };
 
 struct LaunchShape {
@@ -87,6 +91,11 @@ struct CompactParams {
 };
 
 int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu);
+// Orders each band's tiles by the cost its waves measured in the previous frame (most
+// expensive first, in power-of-two classes, stable within a class).
+int launch_tile_order(const uint32_t *tile_cost, uint32_t *order, const uint32_t range_begin[9], hipStream_t stream);
+// Sets n words to zero (a kernel, so it may be captured in a graph).
+int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t stream);
 int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream);
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream);
 // Compositing modes of the trace kernel (camera table, 256-lane work-groups).

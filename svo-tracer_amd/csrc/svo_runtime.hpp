@@ -74,7 +74,13 @@ struct BatchPlan {
 // (a graph keeps its own so later frames never move memory a replay writes to).
 struct FrameScratch {
   std::shared_ptr<DeviceBuffer> status, requests, small, counts, cams;
+  // tile schedule: the per-tile cost of the last frame and the order built from it, valid
+  // for the tiling (band ranges) they were measured on
+  std::shared_ptr<DeviceBuffer> tile_cost, tile_order;
+  uint32_t sched_ranges[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
+// Launches the tile-order kernel of a planned frame (no-op without a schedule).
+int launch_schedule(const BatchPlan &plan, hipStream_t stream);
 int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
                const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
                const float *lut_dev, const uint32_t *lut_offsets, BatchPlan *plan);

@@ -22,6 +22,8 @@
 //     (ascending cone id) by the order-preserving compaction kernels below.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "svo_device.hpp"
 #include "svo_kernels.hpp"
 
@@ -75,6 +77,13 @@ trace_frame_kernel(const FrameParams p) {
   uint32_t range_step = 0;  // wave-uniform: ranges tried, in order xcd, xcd+1, ...
   bool fetching = true;     // wave-uniform
   uint32_t cur = 0, cur_end = 0, cur_t0 = 0;  // wave-uniform: the wave's claimed slice of work
+  // wave-uniform: the tile the wave took at its last refill and the loop iterations since
+  uint32_t cur_tile = kU32Max, age = 0;
+  auto record_cost = [&]() {
+    if (p.tile_cost && cur_tile != kU32Max && lane == 0u) p.tile_cost[cur_tile] = age;
+    cur_tile = kU32Max;
+    age = 0;
+  };
 
   bool active = false;
   uint32_t ray = 0, depth = 0, signs = 0, node_addr = 0, last_group = kU32Max, nvis = 0, nblk = 0;
@@ -256,6 +265,7 @@ trace_frame_kernel(const FrameParams p) {
     const uint32_t n_idle = (uint32_t)__popcll(idle);
     if (fetching && (n_idle >= p.refill || n_idle == 64u)) {
       flush();
+      record_cost();
       // claim a slice of `chunk` work items with one atomic when the wave's slice is used up
       while (cur >= cur_end && range_step < 8u) {
         const uint32_t r = (xcd + range_step) & 7u;
@@ -276,9 +286,11 @@ trace_frame_kernel(const FrameParams p) {
       } else {
         const uint32_t k = cur + lanes_below(idle);
         const uint32_t take = min(n_idle, cur_end - cur);
+        cur_tile = p.order ? p.order[cur_t0 + (cur >> 6)] : cur_t0 + (cur >> 6);
         cur += take;
         if (!active && k < cur_end) {
           uint32_t tile = cur_t0 + (k >> 6);
+          if (p.order) tile = p.order[tile];
           const uint32_t l = k & 63u;
           // the camera whose tiles hold this one: the last with tile_begin <= tile
           uint32_t lo = 0, hi = p.n_cams;
@@ -346,6 +358,7 @@ trace_frame_kernel(const FrameParams p) {
       if (!fetching) break;
       continue;
     }
+    age++;
     if (active) {
       // ---- one step of the reference state machine ----
       // A step that moves to another block only sets node_addr and need_load; the block is
@@ -469,6 +482,7 @@ trace_frame_kernel(const FrameParams p) {
   }
 
   flush();
+  record_cost();
 
   // frame totals: one 64-bit atomic per wave
   uint32_t vb = tot_vis - tot_blk, vk = tot_blk;
@@ -484,6 +498,89 @@ trace_frame_kernel(const FrameParams p) {
     atomicAdd(&p.totals[0], (unsigned long long)vb);
     atomicAdd(&p.totals[1], (unsigned long long)vk);
   }
+}
+
+// ---- tile schedule ----
+
+struct Ranges {
+  uint32_t v[9];
+};
+
+// Cost class of a tile: floor(log2(iterations)), clamped to 15; 0 for an unmeasured tile.
+__device__ __forceinline__ uint32_t cost_class(uint32_t c) { return c ? min(15u, 31u - (uint32_t)__clz(c)) : 0u; }
+
+constexpr uint32_t kOrderThreads = 1024;
+
+// One work-group per band: a stable counting sort of the band's tiles by cost class, most
+// expensive class first, so that the tiles that ran longest in the previous frame start
+// first and the frame does not end waiting on an expensive tile taken late.  Thread t
+// owns a contiguous run of the band's tiles.
+__global__ void __launch_bounds__(kOrderThreads) tile_order_kernel(const uint32_t *cost, uint32_t *order, Ranges rb) {
+  __shared__ uint32_t s_wave[16][kOrderThreads / 64];
+  __shared__ uint32_t s_base[16];
+  const uint32_t b0 = rb.v[blockIdx.x], n = rb.v[blockIdx.x + 1] - b0;
+  const uint32_t per = (n + kOrderThreads - 1) / kOrderThreads;
+  const uint32_t i0 = min(n, threadIdx.x * per), i1 = min(n, i0 + per);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t cnt[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) cnt[c] = 0;
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t k = cost_class(cost[b0 + i]);
+#pragma unroll
+    for (int c = 0; c < 16; c++) cnt[c] += (k == (uint32_t)c) ? 1u : 0u;
+  }
+  // exclusive scan of each class count over the threads, in thread order
+  uint32_t off[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    uint32_t v = cnt[c];
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(v, d);
+      if (lane >= (uint32_t)d) v += t;
+    }
+    off[c] = v - cnt[c];
+    if (lane == 63u) s_wave[c][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16u) {
+    const uint32_t c = threadIdx.x;
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < kOrderThreads / 64; w++) {
+      const uint32_t t = s_wave[c][w];
+      s_wave[c][w] = run;
+      run += t;
+    }
+    s_base[c] = run;  // the class total, for now
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int c = 15; c >= 0; c--) {
+      const uint32_t t = s_base[c];
+      s_base[c] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 16; c++) off[c] += s_base[c] + s_wave[c][wave];
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t k = cost_class(cost[b0 + i]);
+    uint32_t pos = 0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      if (k == (uint32_t)c) {
+        pos = off[c];
+        off[c]++;
+      }
+    }
+    order[b0 + pos] = b0 + i;
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 __global__ void frame_prepare_kernel(uint32_t *counters, unsigned long long *totals) {
@@ -670,6 +767,20 @@ int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
     case 6: return occupancy_waves<6>(wg, lds_bytes, blocks_per_cu);
     default: return -1;
   }
+}
+
+int launch_tile_order(const uint32_t *tile_cost, uint32_t *order, const uint32_t range_begin[9], hipStream_t stream) {
+  Ranges r;
+  for (int i = 0; i < 9; i++) r.v[i] = range_begin[i];
+  hipLaunchKernelGGL(tile_order_kernel, dim3(8), dim3(kOrderThreads), 0, stream, tile_cost, order, r);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t stream) {
+  if (n == 0) return 0;
+  const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 1024u);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream) {

@@ -143,3 +143,23 @@ def test_repeated_frames_identical(svo, orc, gpu):
     b = gpu_frame(svo, gpu, bases, blocks, tree.initial_usages(0), cam)
     for k in ("hit", "visits", "hash", "rgba", "status"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_tile_order_from_previous_frame(svo, orc, gpu):
+    """By default each band's tiles are taken most expensive first, by the cost the previous
+    frame of the same tiling measured.  A frame traced in that order (its output buffers
+    filled with a sentinel first, so a tile left out would show) and a frame in raster order
+    are both bit-exact against the oracle."""
+    B = svo.BufferName
+    spec = svo.scenes.config2_scene(max_depth=10, resolution=512)
+    cam = svo.scenes.config2_camera(12, 320, 200)
+    n = 320 * 200
+    _, ref = run_case(svo, orc, gpu, spec, cam, state=False, queues=False)  # measures the tile costs
+    for flags in (0, svo.FRAME_RASTER_ORDER, 0):
+        gpu.WriteBuffer(B.HitIndices, np.full(n, 0xDEADBEEF, np.uint32))
+        gpu.WriteBuffer(B.VisitCounts, np.full(n, 0xDEADBEEF, np.uint32))
+        gpu.WriteBuffer(B.ChildRequestID, np.zeros(1, np.uint32))
+        gpu.TraceFrame(cam, flags=flags)
+        gpu.Wait()
+        assert np.array_equal(gpu.ReadBuffer(B.HitIndices, np.uint32, n), ref["hit"]), flags
+        assert np.array_equal(gpu.ReadBuffer(B.VisitCounts, np.uint32, n), ref["visits"]), flags
