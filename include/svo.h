@@ -529,6 +529,17 @@ int svo_terrain_create(svo_context *ctx, const svo_geometry *geometry, svo_terra
 int svo_terrain_destroy(svo_terrain *terrain);
 /* Copies the finest height grid (resolution^2 floats, row-major) to the host. */
 int svo_terrain_heights(const svo_terrain *terrain, float *dst, size_t count);
+/* TreeBuilder.BuildTree(baseDepth, maxDepth, maxSize) (TreeBuilder.cs:102-256) of the terrain,
+ * built on the context's device: the same Bases, the same blocks at the same addresses (the
+ * depth-first allocation order from the free-group queue), the same BlockCount and group
+ * parents as svo_tree_build with the terrain's geometry.  The tree becomes the context's
+ * Bases and Blocks (and Usages, as svo_context_load_tree sets them, when with_usages);
+ * *out (optional) receives a host copy, *block_count (optional) the block count.  As there,
+ * a free-group queue that runs out at a dense-level block is SVO_ERR_CAPACITY, and one that
+ * runs out deeper leaves the remaining blocks childless.  base_depth 1..7, max_depth <= 20. */
+int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *terrain, uint32_t base_depth, uint32_t max_depth,
+                           uint32_t max_size, int with_usages, uint16_t usage_tick, svo_tree **out,
+                           uint32_t *block_count);
 /* Seed tree in the bound Bases / Blocks / Usages: Bases of levels 1..base_depth and the dense
  * block level base_depth+2 exactly as svo_tree_build with max_depth = base_depth+2 makes them
  * (BuildBaseChunks / PopulateInitialBlocks, TreeBuilder.cs:152-227); the rest of Blocks is

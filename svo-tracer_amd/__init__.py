@@ -398,6 +398,8 @@ _PROTOS = {
     "svo_terrain_destroy": [C.c_void_p],
     "svo_terrain_heights": [C.c_void_p, C.c_void_p, C.c_size_t],
     "svo_stream_seed": [C.c_void_p, C.c_void_p, C.c_uint32],
+    "svo_terrain_build_tree": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint16,
+                               C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)],
     "svo_stream_update": [C.c_void_p, C.c_void_p, C.POINTER(StreamOptions), C.POINTER(StreamStats)],
 }
 
@@ -631,6 +633,17 @@ class ComputeManager:
         h = C.c_void_p()
         _check(lib().svo_terrain_create(self._h, C.byref(geometry), C.byref(h)), "CreateTerrain")
         return Terrain(h, geometry)
+
+    def BuildTreeOnDevice(self, terrain: "Terrain", base_depth: int, max_depth: int, max_size: int = 0,
+                          with_usages: bool = False, usage_tick: int = 0, host_copy: bool = False):
+        """svo_terrain_build_tree: TreeBuilder.BuildTree of the terrain on the device, bound as
+        this context's Bases / Blocks (/ Usages).  Returns (block_count, Octree or None)."""
+        h = C.c_void_p()
+        count = C.c_uint32(0)
+        _check(lib().svo_terrain_build_tree(self._h, terrain.handle, base_depth, max_depth, max_size,
+                                            1 if with_usages else 0, usage_tick,
+                                            C.byref(h) if host_copy else None, C.byref(count)), "BuildTreeOnDevice")
+        return int(count.value), (Octree(h) if host_copy else None)
 
     def StreamSeed(self, terrain: "Terrain", base_depth: int) -> None:
         """svo_stream_seed: Bases, the dense block level and the usage table of a fresh pool."""

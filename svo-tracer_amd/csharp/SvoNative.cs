@@ -214,6 +214,8 @@ namespace SvoTracer.Kernel.Native
 		[DllImport(Lib)] public static extern int svo_terrain_create(IntPtr ctx, ref NativeGeometry geometry, out IntPtr terrain);
 		[DllImport(Lib)] public static extern int svo_terrain_destroy(IntPtr terrain);
 		[DllImport(Lib)] public static extern int svo_stream_seed(IntPtr ctx, IntPtr terrain, uint baseDepth);
+		[DllImport(Lib)] public static extern int svo_terrain_build_tree(IntPtr ctx, IntPtr terrain, uint baseDepth, uint maxDepth,
+			uint maxSize, int withUsages, ushort usageTick, out IntPtr tree, out uint blockCount);
 		[DllImport(Lib)] public static extern int svo_stream_update(IntPtr ctx, IntPtr terrain, ref NativeStreamOptions options,
 			out NativeStreamStats stats);
 	}
@@ -442,6 +444,16 @@ namespace SvoTracer.Kernel.Native
 
 		public void StreamSeed(IntPtr terrain, uint baseDepth)
 			=> HandleResultCode(SvoNative.svo_stream_seed(_ctx, terrain, baseDepth), "StreamSeed");
+
+		// TreeBuilder.BuildTree of a device terrain, bound as this context's Bases/Blocks(/Usages);
+		// the returned handle is a host copy of the tree (free it with svo_tree_free).
+		public IntPtr BuildTreeOnDevice(IntPtr terrain, uint baseDepth, uint maxDepth, uint maxSize, bool withUsages,
+			ushort usageTick, out uint blockCount)
+		{
+			HandleResultCode(SvoNative.svo_terrain_build_tree(_ctx, terrain, baseDepth, maxDepth, maxSize,
+				withUsages ? 1 : 0, usageTick, out IntPtr tree, out blockCount), "BuildTreeOnDevice");
+			return tree;
+		}
 
 		public NativeStreamStats StreamUpdate(IntPtr terrain, NativeStreamOptions options)
 		{

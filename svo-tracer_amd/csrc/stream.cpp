@@ -381,3 +381,291 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
   if (stats) *stats = out;
   return SVO_OK;
 }
+
+// ---------------------------------------------------------------------------
+// svo_terrain_build_tree: TreeBuilder (BuildTree, TreeBuilder.cs:102-256) on the device
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct BuildLevel {
+  uint32_t n = 0;      // nodes (dense level: x-outer positions, 8^(B+2))
+  uint32_t n_int = 0;  // nodes that get children
+  std::shared_ptr<DeviceBuffer> flag, irank, cells, alloc, writes, rank, start, post;
+};
+
+struct Builder {
+  Context *ctx;
+  hipStream_t s;
+  std::vector<std::shared_ptr<DeviceBuffer>> keep;
+
+  int buf(size_t bytes, std::shared_ptr<DeviceBuffer> *out) { return new_buffer(std::max<size_t>(bytes, 4), out); }
+  template <class T>
+  T *ptr(const std::shared_ptr<DeviceBuffer> &b) {
+    return b ? (T *)b->ptr : nullptr;
+  }
+  int launched(int r, const char *what) {
+    return r ? fail(SVO_ERR_HIP, std::string("svo_terrain_build_tree: launch ") + what) : SVO_OK;
+  }
+  int read(const void *dev, void *host, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? SVO_OK : hip_fail(e, "svo_terrain_build_tree: read back");
+  }
+  // Exclusive ranks of a 0/1 flag array and the number of ones.
+  int rank(const uint32_t *flags, uint32_t n, std::shared_ptr<DeviceBuffer> *out, uint32_t *total) {
+    std::shared_ptr<DeviceBuffer> counts;
+    int rc = buf(((size_t)n / 1024 + 2) * 4, &counts);
+    if (!rc) rc = buf((size_t)n * 4, out);
+    if (rc) return rc;
+    uint32_t *c = ptr<uint32_t>(counts);
+    const uint32_t chunks = (n + 1023) / 1024;
+    if ((rc = launched(launch_rank(flags, n, c, ptr<uint32_t>(*out), c + chunks, s), "rank"))) return rc;
+    *total = 0;
+    return n ? read(c + chunks, total, 4) : SVO_OK;
+  }
+  int fill(const std::shared_ptr<DeviceBuffer> &b, int byte) {
+    hipError_t e = hipMemsetAsync(b->ptr, byte, b->bytes, s);
+    return e == hipSuccess ? SVO_OK : hip_fail(e, "svo_terrain_build_tree: clear");
+  }
+};
+
+}  // namespace
+
+extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, uint32_t base_depth, uint32_t max_depth,
+                                      uint32_t max_size, int with_usages, uint16_t usage_tick, svo_tree **out,
+                                      uint32_t *block_count) {
+  if (!ctx || !t) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: null argument");
+  if (out) *out = nullptr;
+  if (base_depth < 1 || base_depth > 7)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: base_depth must be 1..7");
+  if (max_depth > 20) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: max_depth must be <= 20");
+  if (t->device != ctx->device)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: terrain of another device");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(SVO_ERR_HIP, "hipSetDevice");
+  Builder bd{ctx, ctx->stream, {}};
+  const uint32_t B = base_depth, L0 = B + 2;
+  const uint32_t dense_groups = 1u << (3 * (B + 1)), n0 = 1u << (3 * L0);
+  // FIFO length, as TreeBuilder.BuildTree sizes it (maxSize at least the dense level)
+  const uint64_t dense_min = 1ull << (3 * B + 6);
+  const uint64_t msize = std::max<uint64_t>(max_size, dense_min);
+  const uint64_t range_end = std::max<uint64_t>(msize >> 3, dense_groups);
+  int rc;
+
+  // Bases, and the level-(B+1) cells: mixed ones in PopulateInitialBlocks' write order,
+  // empty ones (ascending) as the head of the free-group FIFO
+  std::shared_ptr<DeviceBuffer> bases, mixed, empty, mixed_rank, empty_rank, empties;
+  if ((rc = bd.buf((size_t)pow_sum(B) * 2, &bases)) || (rc = bd.buf((size_t)dense_groups * 4, &mixed)) ||
+      (rc = bd.buf((size_t)dense_groups * 4, &empty)))
+    return rc;
+  if ((rc = bd.launched(launch_seed_bases(t->view, bd.ptr<uint16_t>(bases), B, bd.s), "bases"))) return rc;
+  if ((rc = bd.launched(launch_bt_octant_flags(bd.ptr<uint16_t>(bases), B, bd.ptr<uint32_t>(mixed),
+                                               bd.ptr<uint32_t>(empty), bd.s),
+                        "octants")))
+    return rc;
+  uint32_t n_mixed = 0, n_empties = 0;
+  if ((rc = bd.rank(bd.ptr<uint32_t>(mixed), dense_groups, &mixed_rank, &n_mixed))) return rc;
+  if ((rc = bd.rank(bd.ptr<uint32_t>(empty), dense_groups, &empty_rank, &n_empties))) return rc;
+  if ((rc = bd.buf((size_t)n_empties * 4, &empties))) return rc;
+  if ((rc = bd.launched(launch_bt_scatter_index(bd.ptr<uint32_t>(empty), bd.ptr<uint32_t>(empty_rank), dense_groups,
+                                                bd.ptr<uint32_t>(empties), bd.s),
+                        "empties")))
+    return rc;
+  const uint64_t capacity64 = (uint64_t)n_empties + (range_end - dense_groups);
+  const uint32_t capacity = (uint32_t)std::min<uint64_t>(capacity64, 0xffffffffu);
+
+  // expansion, level by level: which blocks get children (geometry only)
+  std::vector<BuildLevel> lv(1);
+  lv[0].n = n0;
+  if ((rc = bd.buf((size_t)n0 * 4, &lv[0].flag))) return rc;
+  if ((rc = bd.launched(launch_bt_dense_flags(t->view, bd.ptr<uint16_t>(bases), B, max_depth,
+                                              bd.ptr<uint32_t>(lv[0].flag), bd.s),
+                        "dense flags")))
+    return rc;
+  if ((rc = bd.rank(bd.ptr<uint32_t>(lv[0].flag), n0, &lv[0].irank, &lv[0].n_int))) return rc;
+  if ((rc = bd.buf((size_t)lv[0].n_int * 12, &lv[0].cells))) return rc;
+  if ((rc = bd.launched(launch_bt_dense_cells(bd.ptr<uint32_t>(lv[0].flag), bd.ptr<uint32_t>(lv[0].irank), L0,
+                                              bd.ptr<uint3>(lv[0].cells), bd.s),
+                        "dense cells")))
+    return rc;
+  while (lv.back().n_int > 0) {
+    const BuildLevel &p = lv.back();
+    const uint32_t L = L0 + (uint32_t)lv.size();
+    const uint64_t n = 8ull * p.n_int;
+    if (n > 0xffffffffull) return fail(SVO_ERR_CAPACITY, "svo_terrain_build_tree: a level above 2^32 blocks");
+    BuildLevel c;
+    c.n = (uint32_t)n;
+    if ((rc = bd.buf((size_t)n * 4, &c.flag))) return rc;
+    if ((rc = bd.launched(launch_bt_level_flags(t->view, bd.ptr<uint3>(p.cells), c.n, L, max_depth,
+                                                bd.ptr<uint32_t>(c.flag), bd.s),
+                          "level flags")))
+      return rc;
+    if ((rc = bd.rank(bd.ptr<uint32_t>(c.flag), c.n, &c.irank, &c.n_int))) return rc;
+    if ((rc = bd.buf((size_t)c.n_int * 12, &c.cells))) return rc;
+    if ((rc = bd.launched(launch_bt_level_cells(bd.ptr<uint3>(p.cells), bd.ptr<uint32_t>(c.flag),
+                                                bd.ptr<uint32_t>(c.irank), c.n, bd.ptr<uint3>(c.cells), bd.s),
+                          "level cells")))
+      return rc;
+    lv.push_back(std::move(c));
+  }
+
+  // bottom-up: groups and block writes under every internal block
+  for (size_t i = lv.size(); i-- > 0;) {
+    BuildLevel &l = lv[i];
+    if ((rc = bd.buf((size_t)l.n_int * 4, &l.alloc)) || (rc = bd.buf((size_t)l.n_int * 4, &l.writes))) return rc;
+    const BuildLevel *c = i + 1 < lv.size() ? &lv[i + 1] : nullptr;
+    if ((rc = bd.launched(launch_bt_sizes(l.n_int, c ? bd.ptr<uint32_t>(c->flag) : nullptr,
+                                          c ? bd.ptr<uint32_t>(c->irank) : nullptr,
+                                          c ? bd.ptr<uint32_t>(c->alloc) : nullptr,
+                                          c ? bd.ptr<uint32_t>(c->writes) : nullptr, bd.ptr<uint32_t>(l.alloc),
+                                          bd.ptr<uint32_t>(l.writes), bd.s),
+                          "sizes")))
+      return rc;
+  }
+  // top-down: pre-order ranks (FIFO entries) and write positions; the dense level's
+  // internal blocks are the roots, in x-outer / z-inner order
+  std::shared_ptr<DeviceBuffer> totals;
+  if ((rc = bd.buf(16, &totals))) return rc;
+  uint32_t *tot = bd.ptr<uint32_t>(totals);
+  if ((rc = bd.buf((size_t)lv[0].n_int * 4, &lv[0].rank)) || (rc = bd.buf((size_t)lv[0].n_int * 4, &lv[0].start)))
+    return rc;
+  uint32_t root_total[2] = {0, 0};
+  if (lv[0].n_int) {
+    if ((rc = bd.launched(launch_scan_u32(bd.ptr<uint32_t>(lv[0].alloc), bd.ptr<uint32_t>(lv[0].rank), lv[0].n_int,
+                                          tot, bd.s),
+                          "root ranks")) ||
+        (rc = bd.launched(launch_scan_u32(bd.ptr<uint32_t>(lv[0].writes), bd.ptr<uint32_t>(lv[0].start),
+                                          lv[0].n_int, tot + 1, bd.s),
+                          "root writes")))
+      return rc;
+    if ((rc = bd.read(tot, root_total, 8))) return rc;
+    uint32_t last_rank = 0;
+    if ((rc = bd.read(bd.ptr<uint32_t>(lv[0].rank) + lv[0].n_int - 1, &last_rank, 4))) return rc;
+    // TreeBuilder fails when the FIFO is empty at a dense-level block (TreeBuilder.cs:131-136)
+    if (last_rank >= capacity) return fail(SVO_ERR_CAPACITY, "svo_terrain_build_tree: free address queue exhausted");
+  }
+  for (size_t i = 0; i + 1 < lv.size(); i++) {
+    BuildLevel &l = lv[i], &c = lv[i + 1];
+    if ((rc = bd.buf((size_t)c.n_int * 4, &c.rank)) || (rc = bd.buf((size_t)c.n_int * 4, &c.start)) ||
+        (rc = bd.buf((size_t)c.n * 4, &c.post)))
+      return rc;
+    if ((rc = bd.launched(launch_bt_ranks(l.n_int, bd.ptr<uint32_t>(l.rank), bd.ptr<uint32_t>(l.start),
+                                          bd.ptr<uint32_t>(c.flag), bd.ptr<uint32_t>(c.irank),
+                                          bd.ptr<uint32_t>(c.alloc), bd.ptr<uint32_t>(c.writes),
+                                          bd.ptr<uint32_t>(c.rank), bd.ptr<uint32_t>(c.start),
+                                          bd.ptr<uint32_t>(c.post), bd.s),
+                          "ranks")))
+      return rc;
+  }
+
+  // the blocks: groups in use are the dense level and the FIFO entries that were taken
+  const uint64_t taken = std::min<uint64_t>(root_total[0], capacity);
+  const uint64_t groups64 = dense_groups + (taken > n_empties ? taken - n_empties : 0);
+  if (groups64 * 8 > 0xffffffffull) return fail(SVO_ERR_CAPACITY, "svo_terrain_build_tree: more than 2^32 blocks");
+  const uint32_t n_groups = (uint32_t)groups64;
+  std::shared_ptr<DeviceBuffer> blocks, post, gparent, maxes;
+  if ((rc = bd.buf((size_t)n_groups * 128, &blocks)) || (rc = bd.buf((size_t)n_groups * 32, &post)) ||
+      (rc = bd.buf((size_t)n_groups * 4, &gparent)) || (rc = bd.buf(16, &maxes)))
+    return rc;
+  if ((rc = bd.fill(blocks, 0)) || (rc = bd.fill(post, 0xff)) || (rc = bd.fill(gparent, 0xff)) ||
+      (rc = bd.fill(maxes, 0)))
+    return rc;
+  GroupFifo fifo{bd.ptr<uint32_t>(empties), n_empties, dense_groups, capacity};
+  BuildOut o{bd.ptr<uint4>(blocks), bd.ptr<uint32_t>(post), bd.ptr<uint32_t>(gparent),
+             bd.ptr<unsigned long long>(maxes)};
+  if ((rc = bd.launched(launch_bt_write_dense(t->view, bd.ptr<uint16_t>(bases), B, bd.ptr<uint32_t>(lv[0].flag),
+                                              bd.ptr<uint32_t>(lv[0].irank), bd.ptr<uint32_t>(lv[0].rank),
+                                              bd.ptr<uint32_t>(mixed_rank), fifo, o, bd.s),
+                        "write dense")))
+    return rc;
+  for (size_t i = 1; i < lv.size(); i++) {
+    const BuildLevel &p = lv[i - 1], &c = lv[i];
+    const bool deeper = i + 1 < lv.size() || c.n_int;
+    if ((rc = bd.launched(launch_bt_write_level(t->view, bd.ptr<uint3>(p.cells), bd.ptr<uint32_t>(p.rank), c.n,
+                                                L0 + (uint32_t)i, deeper ? bd.ptr<uint32_t>(c.flag) : nullptr,
+                                                bd.ptr<uint32_t>(c.irank), bd.ptr<uint32_t>(c.rank),
+                                                bd.ptr<uint32_t>(c.post), fifo, o, bd.s),
+                          "write level")))
+      return rc;
+  }
+
+  // BlockCount (TreeBuilder.cs:145, 222, 254).  The dense level counts every block it writes
+  // (max address + 1); BuildBlocks then sets count = a + 1 only when count < a, so an
+  // address written when count equals it leaves count unchanged.  The count therefore ends
+  // at M + 1 or M (M the largest address); which one follows from the run of "records"
+  // (writes above everything written before) that ends at M: the first record of the run
+  // sets count = a + 1, the next (a + 1) finds count equal to it and leaves it, the next
+  // sets it again, and so on.
+  unsigned long long mx[2] = {0, 0};
+  if ((rc = bd.read(bd.ptr<unsigned long long>(maxes), mx, 16))) return rc;
+  const uint32_t M = (uint32_t)mx[0], M1 = (uint32_t)mx[1];
+  uint32_t count = n_mixed ? M1 + 1 : 0u;  // nothing is written without a mixed level-(B+1) cell
+  if (n_mixed && M > M1) {
+    // records among the DFS writes: a is one when every written address above it was
+    // written later (post(a) < post(b) for all written b > a); all addresses above M1 are
+    // DFS writes, so positions compare within one phase
+    uint32_t window = 4096;
+    for (;;) {
+      const uint32_t lo = (M - M1 > window) ? M - window + 1 : M1 + 1;
+      std::vector<uint32_t> pw(M - lo + 1);
+      if ((rc = bd.read(bd.ptr<uint32_t>(post) + lo, pw.data(), pw.size() * 4))) return rc;
+      uint32_t suf = pw.back(), a = M;  // a: lowest record of the run so far
+      while (a > lo) {
+        const uint32_t q = pw[a - 1 - lo];
+        if (q == 0xffffffffu || q >= suf) break;  // a - 1 unwritten or written after a higher address
+        suf = q;
+        a--;
+      }
+      if (a > lo || lo == M1 + 1) {
+        const uint32_t run = M - a + 1;
+        // the run's first record: count becomes a + 1 unless it directly follows the dense
+        // level's count (a == M1 + 1, count already a), in which case it stays a
+        bool plus_one = !(a == M1 + 1);
+        if (run % 2 == 0) plus_one = !plus_one;
+        count = plus_one ? M + 1 : M;
+        break;
+      }
+      window *= 4;  // the run reaches the window's bottom: look further down
+    }
+  }
+  if (block_count) *block_count = count;
+
+  // outputs: the context's Bases / Blocks (/ Usages) and, if asked, a host tree
+  const uint32_t out_groups = (count + 7) / 8;
+  std::shared_ptr<DeviceBuffer> cb, ck;
+  if ((rc = new_buffer((size_t)pow_sum(B) * 2, &cb)) || (rc = new_buffer((size_t)count * 16, &ck))) return rc;
+  cb->flags = SVO_BUFFER_READ_ONLY;
+  ck->flags = SVO_BUFFER_READ_ONLY;
+  hipError_t e = hipMemcpyAsync(cb->ptr, bases->ptr, cb->bytes, hipMemcpyDeviceToDevice, bd.s);
+  if (e == hipSuccess && count) e = hipMemcpyAsync(ck->ptr, blocks->ptr, ck->bytes, hipMemcpyDeviceToDevice, bd.s);
+  if (e != hipSuccess) return hip_fail(e, "svo_terrain_build_tree: copy");
+  std::shared_ptr<DeviceBuffer> cu;
+  if (with_usages) {
+    if ((rc = new_buffer((size_t)out_groups * sizeof(svo_usage), &cu))) return rc;
+    cu->flags = SVO_BUFFER_READ_WRITE;
+    if ((rc = bd.launched(launch_bt_usages(bd.ptr<uint32_t>(gparent), out_groups, dense_groups, usage_tick,
+                                           (svo_usage *)cu->ptr, bd.s),
+                          "usages")))
+      return rc;
+  }
+  if (out) {
+    std::unique_ptr<svo_tree> tree(new svo_tree());
+    tree->t.base_depth = B;
+    tree->t.max_depth = max_depth;
+    tree->t.block_count = count;
+    tree->t.bases.resize(pow_sum(B));
+    tree->t.blocks.resize(count);
+    tree->t.group_parent.resize(out_groups);
+    if ((rc = bd.read(bases->ptr, tree->t.bases.data(), tree->t.bases.size() * 2))) return rc;
+    if (count && (rc = bd.read(blocks->ptr, tree->t.blocks.data(), (size_t)count * 16))) return rc;
+    if (out_groups && (rc = bd.read(gparent->ptr, tree->t.group_parent.data(), (size_t)out_groups * 4))) return rc;
+    *out = tree.release();
+  }
+  e = hipStreamSynchronize(bd.s);
+  if (e != hipSuccess) return hip_fail(e, "svo_terrain_build_tree");
+  ctx->buffers[SVO_BUF_BASES] = cb;
+  ctx->buffers[SVO_BUF_BLOCKS] = ck;
+  if (cu) ctx->buffers[SVO_BUF_USAGES] = cu;
+  ctx->tree_max_depth = max_depth;
+  return SVO_OK;
+}

@@ -368,6 +368,223 @@ __global__ __launch_bounds__(kWg) void identity_kernel(uint32_t *remap, uint32_t
   for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += gridDim.x * kWg) remap[i] = i;
 }
 
+// ---------------------------------------------------------------------------
+// device TreeBuilder (BuildTree, TreeBuilder.cs:102-256) for a terrain.
+//
+// TreeBuilder allocates child groups depth first: the dense level's blocks that need
+// children in x-outer / z-inner order, each block's children 0..7 in order, a child's
+// subtree before its next sibling, every group taken from one FIFO (the sorted empty
+// level-(B+1) cells, then groups 8^(B+1)..).  The k-th allocation (its pre-order rank)
+// therefore gets FIFO entry k.  Here the tree is first expanded level by level (which
+// blocks need children: geometry only), then every internal block's subtree size is
+// summed bottom-up, and the ranks follow top-down as prefix sums of the sizes.  Blocks
+// are written in the same pass; each block's position in TreeBuilder's write sequence
+// (its BlockCount rule depends on that order) is kept for the final count.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool has_children_bits(uint32_t chunk) {  // CanHaveChildren, TreeBuilder.cs:291-302
+  return ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
+}
+
+__device__ __forceinline__ uint32_t spread3_dev(uint32_t v) { return dev::spread3(v); }
+
+// Level-(B+1) cell bp holds both air and geometry in Bases level B.
+__device__ __forceinline__ bool dense_mixed(const uint16_t *bases, uint32_t B, uint32_t bp) {
+  const uint32_t base_start = ((1u << (3u * B)) - 1u) / 7u - 1u;  // powSum(B-1)
+  return ((bases[base_start + (bp >> 3)] >> ((bp & 7u) * 2)) & 3u) == 3u;
+}
+
+// Dense level L0 = B+2 in x-outer / z-inner order: p = (x << 2s) | (y << s) | z.
+// flag[p] = the block is written and gets children.
+__global__ __launch_bounds__(kWg) void bt_dense_flags_kernel(TerrainView t, const uint16_t *bases, uint32_t B,
+                                                             uint32_t max_depth, uint32_t *flag) {
+  const uint32_t L0 = B + 2, n = 1u << (3 * L0), m = (1u << L0) - 1u;
+  for (uint32_t p = blockIdx.x * kWg + threadIdx.x; p < n; p += gridDim.x * kWg) {
+    const uint32_t x = p >> (2 * L0), y = (p >> L0) & m, z = p & m;
+    const uint32_t la = spread3_dev(x) | (spread3_dev(y) << 1) | (spread3_dev(z) << 2);
+    bool f = false;
+    if (L0 < max_depth && dense_mixed(bases, B, la >> 3)) f = has_children_bits(make_chunk(t, L0, x, y, z));
+    flag[p] = f ? 1u : 0u;
+  }
+}
+
+// Level-(B+1) cells in z-outer / x-inner order (PopulateInitialBlocks' loop): mixed[q] with
+// q = (z << 2s) | (y << s) | x; empty[bp] for the free-group FIFO (bp ascending).
+__global__ __launch_bounds__(kWg) void bt_octant_flags_kernel(const uint16_t *bases, uint32_t B, uint32_t *mixed_zyx,
+                                                              uint32_t *empty) {
+  const uint32_t S = B + 1, n = 1u << (3 * S), m = (1u << S) - 1u;
+  for (uint32_t q = blockIdx.x * kWg + threadIdx.x; q < n; q += gridDim.x * kWg) {
+    const uint32_t z = q >> (2 * S), y = (q >> S) & m, x = q & m;
+    const uint32_t bp = spread3_dev(x) | (spread3_dev(y) << 1) | (spread3_dev(z) << 2);
+    const bool mx = dense_mixed(bases, B, bp);
+    mixed_zyx[q] = mx ? 1u : 0u;
+    empty[bp] = mx ? 0u : 1u;
+  }
+}
+
+__global__ __launch_bounds__(kWg) void bt_scatter_index_kernel(const uint32_t *flag, const uint32_t *rank, uint32_t n,
+                                                               uint32_t *out) {
+  for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += gridDim.x * kWg)
+    if (flag[i]) out[rank[i]] = i;
+}
+
+// Cells of the internal blocks of the dense level, from their x-outer positions.
+__global__ __launch_bounds__(kWg) void bt_dense_cells_kernel(const uint32_t *flag, const uint32_t *rank, uint32_t L0,
+                                                             uint3 *cells) {
+  const uint32_t n = 1u << (3 * L0), m = (1u << L0) - 1u;
+  for (uint32_t p = blockIdx.x * kWg + threadIdx.x; p < n; p += gridDim.x * kWg)
+    if (flag[p]) cells[rank[p]] = make_uint3(p >> (2 * L0), (p >> L0) & m, p & m);
+}
+
+__device__ __forceinline__ uint3 child_cell(uint3 c, uint32_t i) {
+  return make_uint3(2u * c.x + (i & 1u), 2u * c.y + ((i >> 1) & 1u), 2u * c.z + ((i >> 2) & 1u));
+}
+
+// Level L (> L0) nodes: node k is child k & 7 of internal block k >> 3 of level L-1.
+__global__ __launch_bounds__(kWg) void bt_level_flags_kernel(TerrainView t, const uint3 *parent_cells, uint32_t n,
+                                                             uint32_t L, uint32_t max_depth, uint32_t *flag) {
+  for (uint32_t k = blockIdx.x * kWg + threadIdx.x; k < n; k += gridDim.x * kWg) {
+    bool f = false;
+    if (L < max_depth) {
+      const uint3 c = child_cell(parent_cells[k >> 3], k & 7u);
+      f = has_children_bits(make_chunk(t, L, c.x, c.y, c.z));
+    }
+    flag[k] = f ? 1u : 0u;
+  }
+}
+
+__global__ __launch_bounds__(kWg) void bt_level_cells_kernel(const uint3 *parent_cells, const uint32_t *flag,
+                                                             const uint32_t *rank, uint32_t n, uint3 *cells) {
+  for (uint32_t k = blockIdx.x * kWg + threadIdx.x; k < n; k += gridDim.x * kWg)
+    if (flag[k]) cells[rank[k]] = child_cell(parent_cells[k >> 3], k & 7u);
+}
+
+// Bottom-up: allocations (groups) and block writes in the subtree of internal block j.
+__global__ __launch_bounds__(kWg) void bt_sizes_kernel(uint32_t n_int, const uint32_t *cflag, const uint32_t *crank,
+                                                       const uint32_t *calloc, const uint32_t *cwrites,
+                                                       uint32_t *alloc, uint32_t *writes) {
+  for (uint32_t j = blockIdx.x * kWg + threadIdx.x; j < n_int; j += gridDim.x * kWg) {
+    uint32_t a = 1, w = 8;
+    if (cflag) {
+      for (uint32_t i = 0; i < 8u; i++) {
+        const uint32_t k = 8u * j + i;
+        if (cflag[k]) {
+          a += calloc[crank[k]];
+          w += cwrites[crank[k]];
+        }
+      }
+    }
+    alloc[j] = a;
+    writes[j] = w;
+  }
+}
+
+// Top-down: the pre-order rank and first write position of each internal child, and the
+// write position of every child block of internal block j.
+__global__ __launch_bounds__(kWg) void bt_ranks_kernel(uint32_t n_int, const uint32_t *rank, const uint32_t *start,
+                                                       const uint32_t *cflag, const uint32_t *crank,
+                                                       const uint32_t *calloc, const uint32_t *cwrites,
+                                                       uint32_t *child_rank, uint32_t *child_start,
+                                                       uint32_t *child_post) {
+  for (uint32_t j = blockIdx.x * kWg + threadIdx.x; j < n_int; j += gridDim.x * kWg) {
+    uint32_t r = rank[j] + 1u, w = start[j];
+    for (uint32_t i = 0; i < 8u; i++) {
+      const uint32_t k = 8u * j + i;
+      if (cflag && cflag[k]) {
+        const uint32_t c = crank[k];
+        child_rank[c] = r;
+        child_start[c] = w;
+        r += calloc[c];
+        w += cwrites[c];
+      }
+      child_post[k] = w;
+      w += 1u;
+    }
+  }
+}
+
+// FIFO entry r: the sorted empty level-(B+1) cells first, then groups dense_groups..
+__device__ __forceinline__ uint32_t fifo_group(const GroupFifo &f, uint32_t r) {
+  return r < f.n_empties ? f.empties[r] : f.dense_groups + (r - f.n_empties);
+}
+
+__device__ __forceinline__ void bt_store(const BuildOut &o, uint32_t a, uint4 b, uint32_t post) {
+  o.blocks[a] = b;
+  o.post[a] = post;
+  atomicMax(&o.max[0], (unsigned long long)a);
+}
+
+// The dense level's blocks (written first, in PopulateInitialBlocks' order).
+__global__ __launch_bounds__(kWg) void bt_write_dense_kernel(TerrainView t, const uint16_t *bases, uint32_t B,
+                                                             const uint32_t *flag, const uint32_t *irank,
+                                                             const uint32_t *rank, const uint32_t *mixed_rank,
+                                                             GroupFifo fifo, BuildOut o) {
+  const uint32_t L0 = B + 2, n = 1u << (3 * L0), m = (1u << L0) - 1u, S = B + 1;
+  for (uint32_t p = blockIdx.x * kWg + threadIdx.x; p < n; p += gridDim.x * kWg) {
+    const uint32_t x = p >> (2 * L0), y = (p >> L0) & m, z = p & m;
+    const uint32_t la = spread3_dev(x) | (spread3_dev(y) << 1) | (spread3_dev(z) << 2);
+    if (!dense_mixed(bases, B, la >> 3)) continue;
+    uint4 b = make_block(t, L0, x, y, z);
+    if (flag[p]) {
+      const uint32_t r = rank[irank[p]];
+      if (r < fifo.capacity) {
+        const uint32_t g = fifo_group(fifo, r);
+        b.x = g << 3;
+        o.group_parent[g] = la;
+      }
+    }
+    const uint32_t q = ((z >> 1) << (2 * S)) | ((y >> 1) << S) | (x >> 1);  // its level-(B+1) cell, z-outer
+    bt_store(o, la, b, 8u * mixed_rank[q] + (la & 7u));
+    atomicMax(&o.max[1], (unsigned long long)la);
+  }
+}
+
+// Level L (> L0) nodes whose parent got its group.
+__global__ __launch_bounds__(kWg) void bt_write_level_kernel(TerrainView t, const uint3 *parent_cells,
+                                                             const uint32_t *parent_rank, uint32_t n, uint32_t L,
+                                                             const uint32_t *flag, const uint32_t *irank,
+                                                             const uint32_t *rank, const uint32_t *post,
+                                                             GroupFifo fifo, BuildOut o) {
+  for (uint32_t k = blockIdx.x * kWg + threadIdx.x; k < n; k += gridDim.x * kWg) {
+    const uint32_t pr = parent_rank[k >> 3];
+    if (pr >= fifo.capacity) continue;  // the FIFO ran out before the parent's group
+    const uint32_t a = (fifo_group(fifo, pr) << 3) + (k & 7u);
+    const uint3 c = child_cell(parent_cells[k >> 3], k & 7u);
+    uint4 b = make_block(t, L, c.x, c.y, c.z);
+    if (flag && flag[k]) {
+      const uint32_t r = rank[irank[k]];
+      if (r < fifo.capacity) {
+        const uint32_t g = fifo_group(fifo, r);
+        b.x = g << 3;
+        o.group_parent[g] = a;
+      }
+    }
+    bt_store(o, a, b, post[k]);
+  }
+}
+
+// Initial usages of a built tree (svo_tree_initial_usages): dense groups inviolate, the
+// allocated ones at `tick` with their parent, the rest free.
+__global__ __launch_bounds__(kWg) void bt_usages_kernel(const uint32_t *group_parent, uint32_t n_groups,
+                                                        uint32_t dense_groups, uint32_t tick, svo_usage *u) {
+  for (uint32_t g = blockIdx.x * kWg + threadIdx.x; g < n_groups; g += gridDim.x * kWg) {
+    const uint32_t parent = group_parent[g];
+    svo_usage v;
+    v.pad = 0;
+    if (g < dense_groups) {
+      v.tick = 65535;
+      v.parent = parent;
+    } else if (parent != kU32Max) {
+      v.tick = (uint16_t)tick;
+      v.parent = parent;
+    } else {
+      v.tick = 0;
+      v.parent = kU32Max;
+    }
+    u[g] = v;
+  }
+}
+
 inline uint32_t grid_for(size_t n) { return (uint32_t)std::min<size_t>((n + kWg - 1) / kWg, 65535 * 16); }
 inline int rc_of() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
@@ -456,6 +673,97 @@ int launch_compact_moves(const PoolView &p, const uint32_t *from, const uint32_t
   hipLaunchKernelGGL(relink_kernel, dim3((8ull * m + kWg - 1) / kWg), dim3(kWg), 0, s, p, from, to, m, remap);
   if (rc_of()) return -1;
   hipLaunchKernelGGL(release_moved_kernel, dim3((m + kWg - 1) / kWg), dim3(kWg), 0, s, p, from, m);
+  return rc_of();
+}
+
+// ---- device TreeBuilder ----
+
+int launch_seed_bases(const TerrainView &t, uint16_t *bases, uint32_t B, hipStream_t s) {
+  for (uint32_t d = 1; d <= B; d++) {
+    const uint32_t offset = ((1u << (3u * d)) - 1u) / 7u - 1u;  // powSum(d-1)
+    hipLaunchKernelGGL(seed_bases_kernel, dim3(grid_for(1ull << (3 * d))), dim3(kWg), 0, s, t, bases, d, offset);
+    if (rc_of()) return -1;
+  }
+  return 0;
+}
+
+int launch_bt_octant_flags(const uint16_t *bases, uint32_t B, uint32_t *mixed_zyx, uint32_t *empty, hipStream_t s) {
+  hipLaunchKernelGGL(bt_octant_flags_kernel, dim3(grid_for(1ull << (3 * (B + 1)))), dim3(kWg), 0, s, bases, B,
+                     mixed_zyx, empty);
+  return rc_of();
+}
+
+int launch_bt_dense_flags(const TerrainView &t, const uint16_t *bases, uint32_t B, uint32_t max_depth, uint32_t *flag,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(bt_dense_flags_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, t, bases, B,
+                     max_depth, flag);
+  return rc_of();
+}
+
+int launch_bt_scatter_index(const uint32_t *flag, const uint32_t *rank, uint32_t n, uint32_t *out, hipStream_t s) {
+  if (!n) return 0;
+  hipLaunchKernelGGL(bt_scatter_index_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, flag, rank, n, out);
+  return rc_of();
+}
+
+int launch_bt_dense_cells(const uint32_t *flag, const uint32_t *rank, uint32_t L0, uint3 *cells, hipStream_t s) {
+  hipLaunchKernelGGL(bt_dense_cells_kernel, dim3(grid_for(1ull << (3 * L0))), dim3(kWg), 0, s, flag, rank, L0, cells);
+  return rc_of();
+}
+
+int launch_bt_level_flags(const TerrainView &t, const uint3 *parent_cells, uint32_t n, uint32_t L, uint32_t max_depth,
+                          uint32_t *flag, hipStream_t s) {
+  if (!n) return 0;
+  hipLaunchKernelGGL(bt_level_flags_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, t, parent_cells, n, L, max_depth, flag);
+  return rc_of();
+}
+
+int launch_bt_level_cells(const uint3 *parent_cells, const uint32_t *flag, const uint32_t *rank, uint32_t n,
+                          uint3 *cells, hipStream_t s) {
+  if (!n) return 0;
+  hipLaunchKernelGGL(bt_level_cells_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, parent_cells, flag, rank, n, cells);
+  return rc_of();
+}
+
+int launch_bt_sizes(uint32_t n_int, const uint32_t *cflag, const uint32_t *crank, const uint32_t *calloc,
+                    const uint32_t *cwrites, uint32_t *alloc, uint32_t *writes, hipStream_t s) {
+  if (!n_int) return 0;
+  hipLaunchKernelGGL(bt_sizes_kernel, dim3(grid_for(n_int)), dim3(kWg), 0, s, n_int, cflag, crank, calloc, cwrites,
+                     alloc, writes);
+  return rc_of();
+}
+
+int launch_bt_ranks(uint32_t n_int, const uint32_t *rank, const uint32_t *start, const uint32_t *cflag,
+                    const uint32_t *crank, const uint32_t *calloc, const uint32_t *cwrites, uint32_t *child_rank,
+                    uint32_t *child_start, uint32_t *child_post, hipStream_t s) {
+  if (!n_int) return 0;
+  hipLaunchKernelGGL(bt_ranks_kernel, dim3(grid_for(n_int)), dim3(kWg), 0, s, n_int, rank, start, cflag, crank, calloc,
+                     cwrites, child_rank, child_start, child_post);
+  return rc_of();
+}
+
+int launch_bt_write_dense(const TerrainView &t, const uint16_t *bases, uint32_t B, const uint32_t *flag,
+                          const uint32_t *irank, const uint32_t *rank, const uint32_t *mixed_rank,
+                          const GroupFifo &fifo, const BuildOut &o, hipStream_t s) {
+  hipLaunchKernelGGL(bt_write_dense_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, t, bases, B, flag,
+                     irank, rank, mixed_rank, fifo, o);
+  return rc_of();
+}
+
+int launch_bt_write_level(const TerrainView &t, const uint3 *parent_cells, const uint32_t *parent_rank, uint32_t n,
+                          uint32_t L, const uint32_t *flag, const uint32_t *irank, const uint32_t *rank,
+                          const uint32_t *post, const GroupFifo &fifo, const BuildOut &o, hipStream_t s) {
+  if (!n) return 0;
+  hipLaunchKernelGGL(bt_write_level_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, t, parent_cells, parent_rank, n, L,
+                     flag, irank, rank, post, fifo, o);
+  return rc_of();
+}
+
+int launch_bt_usages(const uint32_t *group_parent, uint32_t n_groups, uint32_t dense_groups, uint32_t tick,
+                     svo_usage *u, hipStream_t s) {
+  if (!n_groups) return 0;
+  hipLaunchKernelGGL(bt_usages_kernel, dim3(grid_for(n_groups)), dim3(kWg), 0, s, group_parent, n_groups,
+                     dense_groups, tick, u);
   return rc_of();
 }
 
