@@ -384,12 +384,32 @@ def main() -> int:
     t0 = time.perf_counter()
     spec = svo.scenes.config2_scene(max_depth=args.depth)
     tree = spec.build()
+    host_build_s = time.perf_counter() - t0
     log(f"scene {spec.name}: {tree.block_count} blocks ({tree.block_count * 16 / 2**20:.0f} MiB), "
-        f"built in {time.perf_counter() - t0:.1f} s")
+        f"built on the host in {host_build_s:.1f} s")
 
     B = svo.BufferName
     cm = svo.ComputeManager(0)
-    cm.LoadTree(tree, with_usages=True, usage_tick=0)
+    # the same tree built on the device (svo_terrain_build_tree) and bound for the frames;
+    # it must equal the host-built one block for block
+    terrain = cm.CreateTerrain(spec.geometry)
+    build_ms = []
+    for rep in range(4):
+        cm.Wait()
+        t0 = time.perf_counter()
+        count, _ = cm.BuildTreeOnDevice(terrain, spec.base_depth, spec.max_depth, spec.max_size, with_usages=True,
+                                        usage_tick=0)
+        cm.Wait()
+        build_ms.append((time.perf_counter() - t0) * 1e3)
+    terrain.close()
+    identical = count == tree.block_count and np.array_equal(
+        cm.ReadBuffer(B.Blocks, np.uint8), tree.blocks.view(np.uint8).reshape(-1)) and np.array_equal(
+        cm.ReadBuffer(B.Bases, np.uint16), tree.bases)
+    device_build_ms = float(np.median(build_ms[1:]))
+    log(f"scene built on the device in {device_build_ms:.1f} ms (first call {build_ms[0]:.1f} ms), "
+        f"identical to the host build: {identical}")
+    if not identical:
+        raise SystemExit("device-built tree differs from the host-built tree")
     usages0 = cm.ReadBuffer(B.Usages, svo.USAGE_DTYPE)
     cm.InitBuffer(B.ChildRequests, np.zeros(6000 * 40, np.uint8))
     cm.InitBuffer(B.ChildRequestID, np.zeros(1, np.uint32))
@@ -456,6 +476,12 @@ def main() -> int:
                         f"({tree.block_count} x 16-B blocks), {K}-frame seeded orbit",
             "cones_per_step": N,
             "tree_blocks": tree.block_count,
+        },
+        "scene_build": {
+            "host_s": round(host_build_s, 2),
+            "device_ms": round(device_build_ms, 2),
+            "identical": bool(identical),
+            "entry_point": "svo_terrain_build_tree (TreeBuilder.BuildTree on the device)",
         },
         "roofline": {
             "bound": "hbm",
