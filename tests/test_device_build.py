@@ -99,6 +99,12 @@ def test_frame_on_device_built_tree(svo, orc, gpu):
     N = svo.BufferName
     cam = svo.scenes.config3_camera(40, 320, 180, base_depth=B)
     n = 320 * 180
+    for name in (N.VisitHashes, N.RayStatus, N.Locations, N.Depths, N.BackgroundQueue, N.MaterialQueue,
+                 N.Parents, N.ParentResidency):
+        try:  # per-cone outputs an earlier test left bound, at its own size
+            gpu.ReleaseBuffer(name)
+        except svo.SvoError:
+            pass
     for name, size in ((N.OutputImage, 16), (N.HitIndices, 4), (N.VisitCounts, 4)):
         gpu.InitDeviceBuffer(name, n, size)
     gpu.InitBuffer(N.ChildRequests, np.zeros(n * 40, np.uint8))
