@@ -540,6 +540,11 @@ int svo_terrain_heights(const svo_terrain *terrain, float *dst, size_t count);
 int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *terrain, uint32_t base_depth, uint32_t max_depth,
                            uint32_t max_size, int with_usages, uint16_t usage_tick, svo_tree **out,
                            uint32_t *block_count);
+/* The same for any builder geometry (svo_tree_build's): spheres, boxes, the sphere + noise SDF
+ * or a heightfield (whose terrain is made and released inside the call). */
+int svo_tree_build_device(svo_context *ctx, const svo_geometry *geometry, uint32_t base_depth, uint32_t max_depth,
+                          uint32_t max_size, int with_usages, uint16_t usage_tick, svo_tree **out,
+                          uint32_t *block_count);
 /* Seed tree in the bound Bases / Blocks / Usages: Bases of levels 1..base_depth and the dense
  * block level base_depth+2 exactly as svo_tree_build with max_depth = base_depth+2 makes them
  * (BuildBaseChunks / PopulateInitialBlocks, TreeBuilder.cs:152-227); the rest of Blocks is

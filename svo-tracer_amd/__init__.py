@@ -400,6 +400,8 @@ _PROTOS = {
     "svo_stream_seed": [C.c_void_p, C.c_void_p, C.c_uint32],
     "svo_terrain_build_tree": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint16,
                                C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)],
+    "svo_tree_build_device": [C.c_void_p, C.POINTER(Geometry), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint16,
+                              C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)],
     "svo_stream_update": [C.c_void_p, C.c_void_p, C.POINTER(StreamOptions), C.POINTER(StreamStats)],
 }
 
@@ -643,6 +645,17 @@ class ComputeManager:
         _check(lib().svo_terrain_build_tree(self._h, terrain.handle, base_depth, max_depth, max_size,
                                             1 if with_usages else 0, usage_tick,
                                             C.byref(h) if host_copy else None, C.byref(count)), "BuildTreeOnDevice")
+        return int(count.value), (Octree(h) if host_copy else None)
+
+    def BuildTreeDevice(self, geometry: "Geometry", base_depth: int, max_depth: int, max_size: int = 0,
+                        with_usages: bool = False, usage_tick: int = 0, host_copy: bool = False):
+        """svo_tree_build_device: TreeBuilder.BuildTree of any builder geometry on the device, bound
+        as this context's Bases / Blocks (/ Usages).  Returns (block_count, Octree or None)."""
+        h = C.c_void_p()
+        count = C.c_uint32(0)
+        _check(lib().svo_tree_build_device(self._h, C.byref(geometry), base_depth, max_depth, max_size,
+                                           1 if with_usages else 0, usage_tick,
+                                           C.byref(h) if host_copy else None, C.byref(count)), "BuildTreeDevice")
         return int(count.value), (Octree(h) if host_copy else None)
 
     def StreamSeed(self, terrain: "Terrain", base_depth: int) -> None:

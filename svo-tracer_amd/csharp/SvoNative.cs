@@ -216,6 +216,8 @@ namespace SvoTracer.Kernel.Native
 		[DllImport(Lib)] public static extern int svo_stream_seed(IntPtr ctx, IntPtr terrain, uint baseDepth);
 		[DllImport(Lib)] public static extern int svo_terrain_build_tree(IntPtr ctx, IntPtr terrain, uint baseDepth, uint maxDepth,
 			uint maxSize, int withUsages, ushort usageTick, out IntPtr tree, out uint blockCount);
+		[DllImport(Lib)] public static extern int svo_tree_build_device(IntPtr ctx, ref NativeGeometry geometry, uint baseDepth,
+			uint maxDepth, uint maxSize, int withUsages, ushort usageTick, out IntPtr tree, out uint blockCount);
 		[DllImport(Lib)] public static extern int svo_stream_update(IntPtr ctx, IntPtr terrain, ref NativeStreamOptions options,
 			out NativeStreamStats stats);
 	}
@@ -447,6 +449,15 @@ namespace SvoTracer.Kernel.Native
 
 		// TreeBuilder.BuildTree of a device terrain, bound as this context's Bases/Blocks(/Usages);
 		// the returned handle is a host copy of the tree (free it with svo_tree_free).
+		// TreeBuilder.BuildTree of any builder geometry on the device (see BuildTreeOnDevice).
+		public IntPtr BuildTreeDevice(NativeGeometry geometry, uint baseDepth, uint maxDepth, uint maxSize, bool withUsages,
+			ushort usageTick, out uint blockCount)
+		{
+			HandleResultCode(SvoNative.svo_tree_build_device(_ctx, ref geometry, baseDepth, maxDepth, maxSize,
+				withUsages ? 1 : 0, usageTick, out IntPtr tree, out blockCount), "BuildTreeDevice");
+			return tree;
+		}
+
 		public IntPtr BuildTreeOnDevice(IntPtr terrain, uint baseDepth, uint maxDepth, uint maxSize, bool withUsages,
 			ushort usageTick, out uint blockCount)
 		{

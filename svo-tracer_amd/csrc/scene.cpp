@@ -16,94 +16,17 @@
 #include <thread>
 
 #include "svo_internal.hpp"
+#include "svo_shapes.hpp"
 #include "svo_terrain.hpp"
 
 namespace svo {
+shapes::Noise noise_spec(const svo_geometry &g);
+
 namespace {
 
 struct Cell {
   uint32_t level, x, y, z;
 };
-
-// Float bounds of a cell, as BoundingVolume/Location.CoordinateRange compute them
-// (exact for levels <= 24).
-struct Box {
-  float min[3], max[3];
-};
-
-Box cell_box(const Cell &c) {
-  Box b;
-  const float scale = std::ldexp(1.0f, -(int)c.level);
-  const uint32_t v[3] = {c.x, c.y, c.z};
-  for (int a = 0; a < 3; a++) {
-    b.min[a] = (float)v[a] * scale;
-    b.max[a] = (float)(v[a] + 1) * scale;
-  }
-  return b;
-}
-
-// NormalAngle (TreeBuilder.cs:328-334) with the axis convention of the decoder
-// normalVector (wavefront.cl:235-245): n = (cosYaw cosPitch, sinYaw cosPitch, sinPitch).
-void normal_angle(double nx, double ny, double nz, int16_t *pitch, int16_t *yaw) {
-  double len = std::sqrt(nx * nx + ny * ny + nz * nz);
-  if (!(len > 0)) {
-    *pitch = 0;
-    *yaw = 0;
-    return;
-  }
-  nx /= len;
-  ny /= len;
-  nz /= len;
-  double p = std::asin(std::max(-1.0, std::min(1.0, nz))) / M_PI;
-  double y = std::atan2(ny, nx) / M_PI;
-  *pitch = (int16_t)(32767.0 * p);
-  *yaw = (int16_t)(32767.0 * y);
-}
-
-inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  return z ^ (z >> 31);
-}
-
-// Lattice value in [-1, 1) from integer coordinates and a seed (SplitMix64 finaliser).
-inline double lattice(uint64_t seed, int64_t x, int64_t y, int64_t z) {
-  uint64_t h = mix64(seed + 0x9e3779b97f4a7c15ull);
-  h = mix64(h ^ (uint64_t)x * 0xd1b54a32d192ed03ull);
-  h = mix64(h ^ (uint64_t)y * 0xabc98388fb8fac03ull);
-  h = mix64(h ^ (uint64_t)z * 0x8cb92ba72f3d8dd7ull);
-  return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-}
-
-inline double smooth(double t) { return t * t * (3.0 - 2.0 * t); }
-
-double value_noise3(uint64_t seed, double x, double y, double z) {
-  double fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
-  int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
-  double tx = smooth(x - fx), ty = smooth(y - fy), tz = smooth(z - fz);
-  double c[2][2][2];
-  for (int k = 0; k < 2; k++)
-    for (int j = 0; j < 2; j++)
-      for (int i = 0; i < 2; i++) c[k][j][i] = lattice(seed, ix + i, iy + j, iz + k);
-  double r = 0;
-  for (int k = 0; k < 2; k++) {
-    double wy0 = c[k][0][0] + (c[k][0][1] - c[k][0][0]) * tx;
-    double wy1 = c[k][1][0] + (c[k][1][1] - c[k][1][0]) * tx;
-    double wz = wy0 + (wy1 - wy0) * ty;
-    r += (k ? tz : 1.0 - tz) * wz;
-  }
-  return r;
-}
-
-double value_noise2(uint64_t seed, double x, double y) {
-  double fx = std::floor(x), fy = std::floor(y);
-  int64_t ix = (int64_t)fx, iy = (int64_t)fy;
-  double tx = smooth(x - fx), ty = smooth(y - fy);
-  double a = lattice(seed, ix, iy, 0), b = lattice(seed, ix + 1, iy, 0);
-  double c = lattice(seed, ix, iy + 1, 0), d = lattice(seed, ix + 1, iy + 1, 0);
-  double ab = a + (b - a) * tx, cd = c + (d - c) * tx;
-  return ab + (cd - ab) * ty;
-}
 
 // Combined geometry interface (the TreeBuilder.ContainsAir/ContainsGeo/GetNormal/
 // GetColour roles, TreeBuilder.cs:30-66).
@@ -114,82 +37,18 @@ class Geometry {
   virtual void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const = 0;
 };
 
-// SphereDefinition (SphereDefinition.cs) combined by TreeBuilder's rules, in float.
+// SphereDefinition (SphereDefinition.cs) combined by TreeBuilder's rules, in float (the
+// shared rules of svo_shapes.hpp, which the device builder uses too).
 class Spheres : public Geometry {
  public:
-  struct S {
-    float o[3], r, lo[3], hi[3];
-  };
-  std::vector<S> s;
-  explicit Spheres(const float *p, uint32_t n) {
-    for (uint32_t i = 0; i < n; i++) {
-      S q;
-      for (int a = 0; a < 3; a++) q.o[a] = p[4 * i + a];
-      q.r = p[4 * i + 3];
-      for (int a = 0; a < 3; a++) {
-        q.lo[a] = q.o[a] - q.r;
-        q.hi[a] = q.o[a] + q.r;
-      }
-      s.push_back(q);
-    }
-  }
-  static bool within(const S &q, const Box &b) {
-    for (int a = 0; a < 3; a++)
-      if (!(b.min[a] < q.hi[a] && b.max[a] > q.lo[a])) return false;
-    return true;
-  }
-  static bool geo(const S &q, const Box &b) {
-    float d2 = q.r * q.r;
-    for (int a = 0; a < 3; a++) {
-      if (q.o[a] < b.min[a]) {
-        float t = q.o[a] - b.min[a];
-        d2 -= t * t;
-      } else if (q.o[a] > b.max[a]) {
-        float t = q.o[a] - b.max[a];
-        d2 -= t * t;
-      }
-    }
-    return d2 > 0;
-  }
-  static bool air(const S &q, const Box &b) {
-    for (int k = 0; k < 8; k++) {
-      float x = ((k & 1) ? b.max[0] : b.min[0]) - q.o[0];
-      float y = ((k & 2) ? b.max[1] : b.min[1]) - q.o[1];
-      float z = ((k & 4) ? b.max[2] : b.min[2]) - q.o[2];
-      if (std::sqrt((x * x + y * y) + z * z) > q.r) return true;
-    }
-    return false;
-  }
+  std::vector<float> s;  // centre and radius per sphere
+  explicit Spheres(const float *p, uint32_t n) : s(p, p + 4 * (size_t)n) {}
   void classify(const Cell &c, bool *a, bool *g) const override {
-    Box b = cell_box(c);
-    bool any_air = true, any_geo = false;
-    for (const S &q : s) {
-      if (!within(q, b)) continue;
-      if (!air(q, b)) any_air = false;
-      if (geo(q, b)) any_geo = true;
-    }
-    *a = any_air;
-    *g = any_geo;
+    shapes::spheres_classify(s.data(), (uint32_t)(s.size() / 4), shapes::cell_box(c.level, c.x, c.y, c.z), a, g);
   }
   void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
-    Box b = cell_box(c);
-    *pitch = 0;
-    *yaw = 0;
-    rgb[0] = rgb[1] = rgb[2] = 0;
-    bool coloured = false;
-    for (const S &q : s) {
-      if (!within(q, b) || !geo(q, b)) continue;
-      float cx = ((b.max[0] - b.min[0]) / 2) + b.min[0];
-      float cy = ((b.max[1] - b.min[1]) / 2) + b.min[1];
-      float cz = ((b.max[2] - b.min[2]) / 2) + b.min[2];
-      normal_angle(cx - q.o[0], cy - q.o[1], cz - q.o[2], pitch, yaw);
-      if (!coloured) {
-        rgb[0] = 255;  // SphereDefinition.Colour
-        rgb[1] = 0;
-        rgb[2] = 0;
-        coloured = true;
-      }
-    }
+    shapes::spheres_attributes(s.data(), (uint32_t)(s.size() / 4), shapes::cell_box(c.level, c.x, c.y, c.z), pitch,
+                               yaw, rgb);
   }
 };
 
@@ -197,31 +56,10 @@ class Spheres : public Geometry {
 // air unless the cell lies inside one.
 class Cubes : public Geometry {
  public:
-  std::vector<Box> boxes;
-  Cubes(const float *p, uint32_t n) {
-    for (uint32_t i = 0; i < n; i++) {
-      Box b;
-      for (int a = 0; a < 3; a++) {
-        b.min[a] = p[6 * i + a];
-        b.max[a] = p[6 * i + 3 + a];
-      }
-      boxes.push_back(b);
-    }
-  }
+  std::vector<float> b;  // min and max per box
+  Cubes(const float *p, uint32_t n) : b(p, p + 6 * (size_t)n) {}
   void classify(const Cell &c, bool *a, bool *g) const override {
-    Box v = cell_box(c);
-    bool any_geo = false, inside = false;
-    for (const Box &b : boxes) {
-      bool ov = true, in = true;
-      for (int k = 0; k < 3; k++) {
-        if (!(v.min[k] < b.max[k] && v.max[k] > b.min[k])) ov = false;
-        if (!(v.min[k] >= b.min[k] && v.max[k] <= b.max[k])) in = false;
-      }
-      any_geo |= ov;
-      inside |= in;
-    }
-    *g = any_geo;
-    *a = !inside;
+    shapes::cubes_classify(b.data(), (uint32_t)(b.size() / 6), shapes::cell_box(c.level, c.x, c.y, c.z), a, g);
   }
   void attributes(const Cell &, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
     *pitch = 0;
@@ -230,73 +68,42 @@ class Cubes : public Geometry {
   }
 };
 
-uint8_t channel(uint64_t h, int shift, int base) { return (uint8_t)(base + (int)((h >> shift) & 63u)); }
-
-// Unit-sphere SDF with value-noise fBm displacement in world space [-s, s]^3:
-// d(p) = |p| - (1 + A * fbm(p)).  A cell is classified conservatively from the
-// SDF at its centre and the Lipschitz bound of d.
+// Unit-sphere SDF with value-noise fBm displacement (svo_shapes.hpp).
 class SphereNoise : public Geometry {
  public:
-  uint64_t seed;
-  uint32_t octaves;
-  double lac, gain, amp, freq, s, lip;
-  SphereNoise(const svo_geometry &g)
-      : seed(g.seed), octaves(g.octaves ? g.octaves : 3), lac(g.lacunarity > 0 ? g.lacunarity : 2.0),
-        gain(g.gain > 0 ? g.gain : 0.5), amp(g.amplitude), freq(g.frequency > 0 ? g.frequency : 4.0),
-        s(g.half_extent > 0 ? g.half_extent : 1.25) {
-    double sum_a = 0, sum_af = 0, a = 1, f = freq;
-    for (uint32_t o = 0; o < octaves; o++) {
-      sum_a += a;
-      sum_af += a * f;
-      a *= gain;
-      f *= lac;
-    }
-    // |d/dt smoothstep| <= 1.5, lattice range 2, three axes.
-    double noise_lip = 3.0 * std::sqrt(3.0) * sum_af / sum_a;
-    lip = 1.0 + std::fabs(amp) * noise_lip;
-  }
-  double fbm(double x, double y, double z) const {
-    double sum = 0, norm = 0, a = 1, f = freq;
-    for (uint32_t o = 0; o < octaves; o++) {
-      sum += a * value_noise3(seed + o, x * f, y * f, z * f);
-      norm += a;
-      a *= gain;
-      f *= lac;
-    }
-    return sum / norm;
-  }
-  double sdf(double x, double y, double z) const {
-    return std::sqrt(x * x + y * y + z * z) - (1.0 + amp * fbm(x, y, z));
-  }
-  void centre(const Cell &c, double p[3], double *half) const {
-    double size = 2.0 * s * std::ldexp(1.0, -(int)c.level);
-    const uint32_t v[3] = {c.x, c.y, c.z};
-    for (int a = 0; a < 3; a++) p[a] = ((double)v[a] + 0.5) * size - s;
-    *half = 0.5 * size;
-  }
-  void classify(const Cell &c, bool *a, bool *g) const override {
-    double p[3], h;
-    centre(c, p, &h);
-    double d = sdf(p[0], p[1], p[2]);
-    double bound = lip * h * std::sqrt(3.0);
-    *a = d > -bound;
-    *g = d < bound;
-  }
+  shapes::Noise n;
+  explicit SphereNoise(const svo_geometry &g) : n(noise_spec(g)) {}
+  void classify(const Cell &c, bool *a, bool *g) const override { shapes::noise_classify(n, c.level, c.x, c.y, c.z, a, g); }
   void attributes(const Cell &c, int16_t *pitch, int16_t *yaw, uint8_t rgb[3]) const override {
-    double p[3], h;
-    centre(c, p, &h);
-    double gx = sdf(p[0] + h, p[1], p[2]) - sdf(p[0] - h, p[1], p[2]);
-    double gy = sdf(p[0], p[1] + h, p[2]) - sdf(p[0], p[1] - h, p[2]);
-    double gz = sdf(p[0], p[1], p[2] + h) - sdf(p[0], p[1], p[2] - h);
-    normal_angle(gx, gy, gz, pitch, yaw);
-    uint64_t hs = mix64(seed ^ ((uint64_t)c.level << 58) ^ ((uint64_t)c.x << 38) ^ ((uint64_t)c.y << 19) ^ c.z);
-    rgb[0] = channel(hs, 0, 150);
-    rgb[1] = channel(hs, 8, 90);
-    rgb[2] = channel(hs, 16, 60);
+    shapes::noise_attributes(n, c.level, c.x, c.y, c.z, pitch, yaw, rgb);
   }
 };
 
 }  // namespace
+
+// Parameters of an SVO_GEOMETRY_SPHERE_NOISE description (defaults applied), with the
+// Lipschitz bound of its SDF; the device builder uses the same function.
+shapes::Noise noise_spec(const svo_geometry &g) {
+  shapes::Noise n;
+  n.seed = g.seed;
+  n.octaves = g.octaves ? g.octaves : 3;
+  n.lac = g.lacunarity > 0 ? g.lacunarity : 2.0;
+  n.gain = g.gain > 0 ? g.gain : 0.5;
+  n.amp = g.amplitude;
+  n.freq = g.frequency > 0 ? g.frequency : 4.0;
+  n.s = g.half_extent > 0 ? g.half_extent : 1.25;
+  double sum_a = 0, sum_af = 0, a = 1, f = n.freq;
+  for (uint32_t o = 0; o < n.octaves; o++) {
+    sum_a += a;
+    sum_af += a * f;
+    a *= n.gain;
+    f *= n.lac;
+  }
+  // |d/dt smoothstep| <= 1.5, lattice range 2, three axes.
+  const double noise_lip = 3.0 * std::sqrt(3.0) * sum_af / sum_a;
+  n.lip = 1.0 + std::fabs(n.amp) * noise_lip;
+  return n;
+}
 
 // Terrain parameters of an SVO_GEOMETRY_FBM_HEIGHTFIELD description (defaults applied); the
 // device terrain of the streaming path uses the same function.

@@ -18,6 +18,7 @@
 
 namespace svo {
 terrain::Spec terrain_spec(const svo_geometry &g);
+shapes::Noise noise_spec(const svo_geometry &g);
 }
 
 using namespace svo;
@@ -432,17 +433,10 @@ struct Builder {
 
 }  // namespace
 
-extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, uint32_t base_depth, uint32_t max_depth,
-                                      uint32_t max_size, int with_usages, uint16_t usage_tick, svo_tree **out,
-                                      uint32_t *block_count) {
-  if (!ctx || !t) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: null argument");
-  if (out) *out = nullptr;
-  if (base_depth < 1 || base_depth > 7)
-    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: base_depth must be 1..7");
-  if (max_depth > 20) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: max_depth must be <= 20");
-  if (t->device != ctx->device)
-    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: terrain of another device");
-  if (hipSetDevice(ctx->device) != hipSuccess) return fail(SVO_ERR_HIP, "hipSetDevice");
+namespace {
+
+int build_tree_device(Context *ctx, const GeomView &g, uint32_t base_depth, uint32_t max_depth, uint32_t max_size,
+                      int with_usages, uint16_t usage_tick, svo_tree **out, uint32_t *block_count) {
   Builder bd{ctx, ctx->stream, {}};
   const uint32_t B = base_depth, L0 = B + 2;
   const uint32_t dense_groups = 1u << (3 * (B + 1)), n0 = 1u << (3 * L0);
@@ -458,7 +452,7 @@ extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, ui
   if ((rc = bd.buf((size_t)pow_sum(B) * 2, &bases)) || (rc = bd.buf((size_t)dense_groups * 4, &mixed)) ||
       (rc = bd.buf((size_t)dense_groups * 4, &empty)))
     return rc;
-  if ((rc = bd.launched(launch_seed_bases(t->view, bd.ptr<uint16_t>(bases), B, bd.s), "bases"))) return rc;
+  if ((rc = bd.launched(launch_bt_bases(g, bd.ptr<uint16_t>(bases), B, bd.s), "bases"))) return rc;
   if ((rc = bd.launched(launch_bt_octant_flags(bd.ptr<uint16_t>(bases), B, bd.ptr<uint32_t>(mixed),
                                                bd.ptr<uint32_t>(empty), bd.s),
                         "octants")))
@@ -478,7 +472,7 @@ extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, ui
   std::vector<BuildLevel> lv(1);
   lv[0].n = n0;
   if ((rc = bd.buf((size_t)n0 * 4, &lv[0].flag))) return rc;
-  if ((rc = bd.launched(launch_bt_dense_flags(t->view, bd.ptr<uint16_t>(bases), B, max_depth,
+  if ((rc = bd.launched(launch_bt_dense_flags(g, bd.ptr<uint16_t>(bases), B, max_depth,
                                               bd.ptr<uint32_t>(lv[0].flag), bd.s),
                         "dense flags")))
     return rc;
@@ -496,7 +490,7 @@ extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, ui
     BuildLevel c;
     c.n = (uint32_t)n;
     if ((rc = bd.buf((size_t)n * 4, &c.flag))) return rc;
-    if ((rc = bd.launched(launch_bt_level_flags(t->view, bd.ptr<uint3>(p.cells), c.n, L, max_depth,
+    if ((rc = bd.launched(launch_bt_level_flags(g, bd.ptr<uint3>(p.cells), c.n, L, max_depth,
                                                 bd.ptr<uint32_t>(c.flag), bd.s),
                           "level flags")))
       return rc;
@@ -573,7 +567,7 @@ extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, ui
   GroupFifo fifo{bd.ptr<uint32_t>(empties), n_empties, dense_groups, capacity};
   BuildOut o{bd.ptr<uint4>(blocks), bd.ptr<uint32_t>(post), bd.ptr<uint32_t>(gparent),
              bd.ptr<unsigned long long>(maxes)};
-  if ((rc = bd.launched(launch_bt_write_dense(t->view, bd.ptr<uint16_t>(bases), B, bd.ptr<uint32_t>(lv[0].flag),
+  if ((rc = bd.launched(launch_bt_write_dense(g, bd.ptr<uint16_t>(bases), B, bd.ptr<uint32_t>(lv[0].flag),
                                               bd.ptr<uint32_t>(lv[0].irank), bd.ptr<uint32_t>(lv[0].rank),
                                               bd.ptr<uint32_t>(mixed_rank), fifo, o, bd.s),
                         "write dense")))
@@ -581,7 +575,7 @@ extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, ui
   for (size_t i = 1; i < lv.size(); i++) {
     const BuildLevel &p = lv[i - 1], &c = lv[i];
     const bool deeper = i + 1 < lv.size() || c.n_int;
-    if ((rc = bd.launched(launch_bt_write_level(t->view, bd.ptr<uint3>(p.cells), bd.ptr<uint32_t>(p.rank), c.n,
+    if ((rc = bd.launched(launch_bt_write_level(g, bd.ptr<uint3>(p.cells), bd.ptr<uint32_t>(p.rank), c.n,
                                                 L0 + (uint32_t)i, deeper ? bd.ptr<uint32_t>(c.flag) : nullptr,
                                                 bd.ptr<uint32_t>(c.irank), bd.ptr<uint32_t>(c.rank),
                                                 bd.ptr<uint32_t>(c.post), fifo, o, bd.s),
@@ -668,4 +662,67 @@ extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, ui
   if (cu) ctx->buffers[SVO_BUF_USAGES] = cu;
   ctx->tree_max_depth = max_depth;
   return SVO_OK;
+}
+
+int check_build_args(svo_context *ctx, uint32_t base_depth, uint32_t max_depth, svo_tree **out, const char *what) {
+  if (out) *out = nullptr;
+  if (base_depth < 1 || base_depth > 7) return fail(SVO_ERR_INVALID_ARGUMENT, std::string(what) + ": base_depth must be 1..7");
+  if (max_depth > 20) return fail(SVO_ERR_INVALID_ARGUMENT, std::string(what) + ": max_depth must be <= 20");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(SVO_ERR_HIP, "hipSetDevice");
+  return SVO_OK;
+}
+
+}  // namespace
+
+extern "C" int svo_terrain_build_tree(svo_context *ctx, const svo_terrain *t, uint32_t base_depth, uint32_t max_depth,
+                                      uint32_t max_size, int with_usages, uint16_t usage_tick, svo_tree **out,
+                                      uint32_t *block_count) {
+  if (!ctx || !t) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: null argument");
+  int rc = check_build_args(ctx, base_depth, max_depth, out, "svo_terrain_build_tree");
+  if (rc) return rc;
+  if (t->device != ctx->device)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_terrain_build_tree: terrain of another device");
+  GeomView g{};
+  g.kind = SVO_GEOMETRY_FBM_HEIGHTFIELD;
+  g.t = t->view;
+  return build_tree_device(ctx, g, base_depth, max_depth, max_size, with_usages, usage_tick, out, block_count);
+}
+
+extern "C" int svo_tree_build_device(svo_context *ctx, const svo_geometry *geometry, uint32_t base_depth,
+                                     uint32_t max_depth, uint32_t max_size, int with_usages, uint16_t usage_tick,
+                                     svo_tree **out, uint32_t *block_count) {
+  if (!ctx || !geometry) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build_device: null argument");
+  int rc = check_build_args(ctx, base_depth, max_depth, out, "svo_tree_build_device");
+  if (rc) return rc;
+  GeomView g{};
+  g.kind = geometry->kind;
+  switch (geometry->kind) {
+    case SVO_GEOMETRY_FBM_HEIGHTFIELD: {
+      svo_terrain *t = nullptr;
+      if ((rc = svo_terrain_create(ctx, geometry, &t))) return rc;
+      g.t = t->view;
+      rc = build_tree_device(ctx, g, base_depth, max_depth, max_size, with_usages, usage_tick, out, block_count);
+      svo_terrain_destroy(t);
+      return rc;
+    }
+    case SVO_GEOMETRY_SPHERES:
+    case SVO_GEOMETRY_CUBES: {
+      const size_t per = geometry->kind == SVO_GEOMETRY_SPHERES ? 4 : 6;
+      if (!geometry->shapes && geometry->count) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build_device: no shapes");
+      std::shared_ptr<DeviceBuffer> shp;
+      if ((rc = new_buffer(std::max<size_t>(per * geometry->count * 4, 4), &shp))) return rc;
+      if (geometry->count) {
+        hipError_t e = hipMemcpy(shp->ptr, geometry->shapes, per * geometry->count * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "svo_tree_build_device: shapes upload");
+      }
+      g.shapes = (const float *)shp->ptr;
+      g.count = geometry->count;
+      return build_tree_device(ctx, g, base_depth, max_depth, max_size, with_usages, usage_tick, out, block_count);
+    }
+    case SVO_GEOMETRY_SPHERE_NOISE:
+      g.noise = noise_spec(*geometry);
+      return build_tree_device(ctx, g, base_depth, max_depth, max_size, with_usages, usage_tick, out, block_count);
+    default:
+      return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build_device: unknown geometry kind");
+  }
 }

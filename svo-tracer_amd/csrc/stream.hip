@@ -10,6 +10,7 @@
 
 #include "svo_device.hpp"
 #include "svo_stream_kernels.hpp"
+#include "svo_shapes.hpp"
 #include "svo_terrain.hpp"
 
 namespace svo {
@@ -70,6 +71,56 @@ __device__ __forceinline__ uint32_t compact3(uint32_t v) {  // every third bit -
   v = (v | (v >> 8)) & 0x030000ffu;
   v = (v | (v >> 16)) & 0x000003ffu;
   return v;
+}
+
+// Any builder geometry: one cell's classification and one block (MakeChunk / MakeBlock,
+// TreeBuilder.cs:74-93, 264-284), by the rules the host builder shares.
+__device__ void geom_classify(const GeomView &g, uint32_t level, uint32_t x, uint32_t y, uint32_t z, bool *air,
+                              bool *geo) {
+  switch (g.kind) {
+    case SVO_GEOMETRY_FBM_HEIGHTFIELD: {
+      float lo, hi;
+      footprint(g.t, level, x, y, &lo, &hi);
+      terrain::classify(level, z, lo, hi, air, geo);
+      return;
+    }
+    case SVO_GEOMETRY_SPHERES:
+      shapes::spheres_classify(g.shapes, g.count, shapes::cell_box(level, x, y, z), air, geo);
+      return;
+    case SVO_GEOMETRY_CUBES:
+      shapes::cubes_classify(g.shapes, g.count, shapes::cell_box(level, x, y, z), air, geo);
+      return;
+    default:
+      shapes::noise_classify(g.noise, level, x, y, z, air, geo);
+  }
+}
+
+__device__ uint16_t geom_chunk(const GeomView &g, uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
+  uint16_t chunk = 0;
+  for (uint32_t k = 0; k < 8; k++) {
+    bool a, e;
+    geom_classify(g, level + 1, 2 * x + (k & 1u), 2 * y + ((k >> 1) & 1u), 2 * z + ((k >> 2) & 1u), &a, &e);
+    chunk |= (uint16_t)((a ? 1u : 0u) << (2 * k));
+    chunk |= (uint16_t)((e ? 1u : 0u) << (2 * k + 1));
+  }
+  return chunk;
+}
+
+__device__ uint4 geom_block(const GeomView &g, uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
+  if (g.kind == SVO_GEOMETRY_FBM_HEIGHTFIELD) return make_block(g.t, level, x, y, z);
+  const uint16_t chunk = geom_chunk(g, level, x, y, z);
+  int16_t pitch = 0, yaw = 0;
+  uint8_t rgb[3] = {255, 255, 255};  // boxes: white, no normal
+  if (g.kind == SVO_GEOMETRY_SPHERES)
+    shapes::spheres_attributes(g.shapes, g.count, shapes::cell_box(level, x, y, z), &pitch, &yaw, rgb);
+  else if (g.kind != SVO_GEOMETRY_CUBES)
+    shapes::noise_attributes(g.noise, level, x, y, z, &pitch, &yaw, rgb);
+  uint4 b;
+  b.x = kU32Max;
+  b.y = (uint32_t)chunk | ((uint32_t)(uint16_t)pitch << 16);
+  b.z = (uint32_t)(uint16_t)yaw | ((uint32_t)rgb[0] << 16) | ((uint32_t)rgb[2] << 24);
+  b.w = (uint32_t)rgb[1] | (255u << 8);
+  return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +420,7 @@ __global__ __launch_bounds__(kWg) void identity_kernel(uint32_t *remap, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// device TreeBuilder (BuildTree, TreeBuilder.cs:102-256) for a terrain.
+// device TreeBuilder (BuildTree, TreeBuilder.cs:102-256) for any builder geometry.
 //
 // TreeBuilder allocates child groups depth first: the dense level's blocks that need
 // children in x-outer / z-inner order, each block's children 0..7 in order, a child's
@@ -388,6 +439,12 @@ __device__ __forceinline__ bool has_children_bits(uint32_t chunk) {  // CanHaveC
 
 __device__ __forceinline__ uint32_t spread3_dev(uint32_t v) { return dev::spread3(v); }
 
+__global__ __launch_bounds__(kWg) void bt_bases_kernel(GeomView g, uint16_t *bases, uint32_t level, uint32_t offset) {
+  const uint32_t n = 1u << (3 * level);
+  for (uint32_t m = blockIdx.x * kWg + threadIdx.x; m < n; m += gridDim.x * kWg)
+    bases[offset + m] = geom_chunk(g, level, compact3(m), compact3(m >> 1), compact3(m >> 2));
+}
+
 // Level-(B+1) cell bp holds both air and geometry in Bases level B.
 __device__ __forceinline__ bool dense_mixed(const uint16_t *bases, uint32_t B, uint32_t bp) {
   const uint32_t base_start = ((1u << (3u * B)) - 1u) / 7u - 1u;  // powSum(B-1)
@@ -396,14 +453,14 @@ __device__ __forceinline__ bool dense_mixed(const uint16_t *bases, uint32_t B, u
 
 // Dense level L0 = B+2 in x-outer / z-inner order: p = (x << 2s) | (y << s) | z.
 // flag[p] = the block is written and gets children.
-__global__ __launch_bounds__(kWg) void bt_dense_flags_kernel(TerrainView t, const uint16_t *bases, uint32_t B,
+__global__ __launch_bounds__(kWg) void bt_dense_flags_kernel(GeomView g, const uint16_t *bases, uint32_t B,
                                                              uint32_t max_depth, uint32_t *flag) {
   const uint32_t L0 = B + 2, n = 1u << (3 * L0), m = (1u << L0) - 1u;
   for (uint32_t p = blockIdx.x * kWg + threadIdx.x; p < n; p += gridDim.x * kWg) {
     const uint32_t x = p >> (2 * L0), y = (p >> L0) & m, z = p & m;
     const uint32_t la = spread3_dev(x) | (spread3_dev(y) << 1) | (spread3_dev(z) << 2);
     bool f = false;
-    if (L0 < max_depth && dense_mixed(bases, B, la >> 3)) f = has_children_bits(make_chunk(t, L0, x, y, z));
+    if (L0 < max_depth && dense_mixed(bases, B, la >> 3)) f = has_children_bits(geom_chunk(g, L0, x, y, z));
     flag[p] = f ? 1u : 0u;
   }
 }
@@ -441,13 +498,13 @@ __device__ __forceinline__ uint3 child_cell(uint3 c, uint32_t i) {
 }
 
 // Level L (> L0) nodes: node k is child k & 7 of internal block k >> 3 of level L-1.
-__global__ __launch_bounds__(kWg) void bt_level_flags_kernel(TerrainView t, const uint3 *parent_cells, uint32_t n,
+__global__ __launch_bounds__(kWg) void bt_level_flags_kernel(GeomView g, const uint3 *parent_cells, uint32_t n,
                                                              uint32_t L, uint32_t max_depth, uint32_t *flag) {
   for (uint32_t k = blockIdx.x * kWg + threadIdx.x; k < n; k += gridDim.x * kWg) {
     bool f = false;
     if (L < max_depth) {
       const uint3 c = child_cell(parent_cells[k >> 3], k & 7u);
-      f = has_children_bits(make_chunk(t, L, c.x, c.y, c.z));
+      f = has_children_bits(geom_chunk(g, L, c.x, c.y, c.z));
     }
     flag[k] = f ? 1u : 0u;
   }
@@ -515,7 +572,7 @@ __device__ __forceinline__ void bt_store(const BuildOut &o, uint32_t a, uint4 b,
 }
 
 // The dense level's blocks (written first, in PopulateInitialBlocks' order).
-__global__ __launch_bounds__(kWg) void bt_write_dense_kernel(TerrainView t, const uint16_t *bases, uint32_t B,
+__global__ __launch_bounds__(kWg) void bt_write_dense_kernel(GeomView g, const uint16_t *bases, uint32_t B,
                                                              const uint32_t *flag, const uint32_t *irank,
                                                              const uint32_t *rank, const uint32_t *mixed_rank,
                                                              GroupFifo fifo, BuildOut o) {
@@ -524,7 +581,7 @@ __global__ __launch_bounds__(kWg) void bt_write_dense_kernel(TerrainView t, cons
     const uint32_t x = p >> (2 * L0), y = (p >> L0) & m, z = p & m;
     const uint32_t la = spread3_dev(x) | (spread3_dev(y) << 1) | (spread3_dev(z) << 2);
     if (!dense_mixed(bases, B, la >> 3)) continue;
-    uint4 b = make_block(t, L0, x, y, z);
+    uint4 b = geom_block(g, L0, x, y, z);
     if (flag[p]) {
       const uint32_t r = rank[irank[p]];
       if (r < fifo.capacity) {
@@ -540,7 +597,7 @@ __global__ __launch_bounds__(kWg) void bt_write_dense_kernel(TerrainView t, cons
 }
 
 // Level L (> L0) nodes whose parent got its group.
-__global__ __launch_bounds__(kWg) void bt_write_level_kernel(TerrainView t, const uint3 *parent_cells,
+__global__ __launch_bounds__(kWg) void bt_write_level_kernel(GeomView g, const uint3 *parent_cells,
                                                              const uint32_t *parent_rank, uint32_t n, uint32_t L,
                                                              const uint32_t *flag, const uint32_t *irank,
                                                              const uint32_t *rank, const uint32_t *post,
@@ -550,7 +607,7 @@ __global__ __launch_bounds__(kWg) void bt_write_level_kernel(TerrainView t, cons
     if (pr >= fifo.capacity) continue;  // the FIFO ran out before the parent's group
     const uint32_t a = (fifo_group(fifo, pr) << 3) + (k & 7u);
     const uint3 c = child_cell(parent_cells[k >> 3], k & 7u);
-    uint4 b = make_block(t, L, c.x, c.y, c.z);
+    uint4 b = geom_block(g, L, c.x, c.y, c.z);
     if (flag && flag[k]) {
       const uint32_t r = rank[irank[k]];
       if (r < fifo.capacity) {
@@ -678,10 +735,10 @@ int launch_compact_moves(const PoolView &p, const uint32_t *from, const uint32_t
 
 // ---- device TreeBuilder ----
 
-int launch_seed_bases(const TerrainView &t, uint16_t *bases, uint32_t B, hipStream_t s) {
+int launch_bt_bases(const GeomView &g, uint16_t *bases, uint32_t B, hipStream_t s) {
   for (uint32_t d = 1; d <= B; d++) {
     const uint32_t offset = ((1u << (3u * d)) - 1u) / 7u - 1u;  // powSum(d-1)
-    hipLaunchKernelGGL(seed_bases_kernel, dim3(grid_for(1ull << (3 * d))), dim3(kWg), 0, s, t, bases, d, offset);
+    hipLaunchKernelGGL(bt_bases_kernel, dim3(grid_for(1ull << (3 * d))), dim3(kWg), 0, s, g, bases, d, offset);
     if (rc_of()) return -1;
   }
   return 0;
@@ -693,9 +750,9 @@ int launch_bt_octant_flags(const uint16_t *bases, uint32_t B, uint32_t *mixed_zy
   return rc_of();
 }
 
-int launch_bt_dense_flags(const TerrainView &t, const uint16_t *bases, uint32_t B, uint32_t max_depth, uint32_t *flag,
+int launch_bt_dense_flags(const GeomView &g, const uint16_t *bases, uint32_t B, uint32_t max_depth, uint32_t *flag,
                           hipStream_t s) {
-  hipLaunchKernelGGL(bt_dense_flags_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, t, bases, B,
+  hipLaunchKernelGGL(bt_dense_flags_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, g, bases, B,
                      max_depth, flag);
   return rc_of();
 }
@@ -711,10 +768,10 @@ int launch_bt_dense_cells(const uint32_t *flag, const uint32_t *rank, uint32_t L
   return rc_of();
 }
 
-int launch_bt_level_flags(const TerrainView &t, const uint3 *parent_cells, uint32_t n, uint32_t L, uint32_t max_depth,
+int launch_bt_level_flags(const GeomView &g, const uint3 *parent_cells, uint32_t n, uint32_t L, uint32_t max_depth,
                           uint32_t *flag, hipStream_t s) {
   if (!n) return 0;
-  hipLaunchKernelGGL(bt_level_flags_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, t, parent_cells, n, L, max_depth, flag);
+  hipLaunchKernelGGL(bt_level_flags_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, g, parent_cells, n, L, max_depth, flag);
   return rc_of();
 }
 
@@ -742,19 +799,19 @@ int launch_bt_ranks(uint32_t n_int, const uint32_t *rank, const uint32_t *start,
   return rc_of();
 }
 
-int launch_bt_write_dense(const TerrainView &t, const uint16_t *bases, uint32_t B, const uint32_t *flag,
+int launch_bt_write_dense(const GeomView &g, const uint16_t *bases, uint32_t B, const uint32_t *flag,
                           const uint32_t *irank, const uint32_t *rank, const uint32_t *mixed_rank,
                           const GroupFifo &fifo, const BuildOut &o, hipStream_t s) {
-  hipLaunchKernelGGL(bt_write_dense_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, t, bases, B, flag,
+  hipLaunchKernelGGL(bt_write_dense_kernel, dim3(grid_for(1ull << (3 * (B + 2)))), dim3(kWg), 0, s, g, bases, B, flag,
                      irank, rank, mixed_rank, fifo, o);
   return rc_of();
 }
 
-int launch_bt_write_level(const TerrainView &t, const uint3 *parent_cells, const uint32_t *parent_rank, uint32_t n,
+int launch_bt_write_level(const GeomView &g, const uint3 *parent_cells, const uint32_t *parent_rank, uint32_t n,
                           uint32_t L, const uint32_t *flag, const uint32_t *irank, const uint32_t *rank,
                           const uint32_t *post, const GroupFifo &fifo, const BuildOut &o, hipStream_t s) {
   if (!n) return 0;
-  hipLaunchKernelGGL(bt_write_level_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, t, parent_cells, parent_rank, n, L,
+  hipLaunchKernelGGL(bt_write_level_kernel, dim3(grid_for(n)), dim3(kWg), 0, s, g, parent_cells, parent_rank, n, L,
                      flag, irank, rank, post, fifo, o);
   return rc_of();
 }

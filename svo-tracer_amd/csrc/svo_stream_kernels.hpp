@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "svo.h"
+#include "svo_shapes.hpp"
 #include "svo_terrain.hpp"
 #include "svo_tree_kernels.hpp"
 
@@ -51,7 +52,16 @@ int launch_select_scatter(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t
 int launch_compact_moves(const PoolView &p, const uint32_t *from, const uint32_t *to, uint32_t m, uint32_t *remap,
                          hipStream_t s);
 
-// ---- device TreeBuilder (BuildTree, TreeBuilder.cs:102-256) over a terrain ----
+// ---- device TreeBuilder (BuildTree, TreeBuilder.cs:102-256) ----
+// The builder's geometry: a terrain, spheres (centre, radius), boxes (min, max) or the
+// sphere-plus-noise SDF, classified by the rules the host builder shares.
+struct GeomView {
+  uint32_t kind;  // SVO_GEOMETRY_*
+  TerrainView t;
+  const float *shapes;
+  uint32_t count;
+  shapes::Noise noise;
+};
 // Free-group FIFO: the n_empties sorted empty level-(B+1) cells, then dense_groups, ... up
 // to `capacity` entries in all.
 struct GroupFifo {
@@ -65,16 +75,16 @@ struct BuildOut {
   uint32_t *group_parent;   // per group: the block whose children it holds
   unsigned long long *max;  // [0] largest address written, [1] largest dense-level address written
 };
-int launch_seed_bases(const TerrainView &t, uint16_t *bases, uint32_t B, hipStream_t s);
+int launch_bt_bases(const GeomView &g, uint16_t *bases, uint32_t B, hipStream_t s);
 // Level-(B+1) cells: mixed_zyx in z-outer / x-inner order, empty by Morton index.
 int launch_bt_octant_flags(const uint16_t *bases, uint32_t B, uint32_t *mixed_zyx, uint32_t *empty, hipStream_t s);
 // Dense level in x-outer / z-inner order: the written blocks that get children.
-int launch_bt_dense_flags(const TerrainView &t, const uint16_t *bases, uint32_t B, uint32_t max_depth, uint32_t *flag,
+int launch_bt_dense_flags(const GeomView &g, const uint16_t *bases, uint32_t B, uint32_t max_depth, uint32_t *flag,
                           hipStream_t s);
 int launch_bt_scatter_index(const uint32_t *flag, const uint32_t *rank, uint32_t n, uint32_t *out, hipStream_t s);
 int launch_bt_dense_cells(const uint32_t *flag, const uint32_t *rank, uint32_t L0, uint3 *cells, hipStream_t s);
 // Level L's nodes (child k & 7 of internal block k >> 3 of level L-1): those that get children.
-int launch_bt_level_flags(const TerrainView &t, const uint3 *parent_cells, uint32_t n, uint32_t L, uint32_t max_depth,
+int launch_bt_level_flags(const GeomView &g, const uint3 *parent_cells, uint32_t n, uint32_t L, uint32_t max_depth,
                           uint32_t *flag, hipStream_t s);
 int launch_bt_level_cells(const uint3 *parent_cells, const uint32_t *flag, const uint32_t *rank, uint32_t n,
                           uint3 *cells, hipStream_t s);
@@ -85,10 +95,10 @@ int launch_bt_sizes(uint32_t n_int, const uint32_t *cflag, const uint32_t *crank
 int launch_bt_ranks(uint32_t n_int, const uint32_t *rank, const uint32_t *start, const uint32_t *cflag,
                     const uint32_t *crank, const uint32_t *calloc, const uint32_t *cwrites, uint32_t *child_rank,
                     uint32_t *child_start, uint32_t *child_post, hipStream_t s);
-int launch_bt_write_dense(const TerrainView &t, const uint16_t *bases, uint32_t B, const uint32_t *flag,
+int launch_bt_write_dense(const GeomView &g, const uint16_t *bases, uint32_t B, const uint32_t *flag,
                           const uint32_t *irank, const uint32_t *rank, const uint32_t *mixed_rank,
                           const GroupFifo &fifo, const BuildOut &o, hipStream_t s);
-int launch_bt_write_level(const TerrainView &t, const uint3 *parent_cells, const uint32_t *parent_rank, uint32_t n,
+int launch_bt_write_level(const GeomView &g, const uint3 *parent_cells, const uint32_t *parent_rank, uint32_t n,
                           uint32_t L, const uint32_t *flag, const uint32_t *irank, const uint32_t *rank,
                           const uint32_t *post, const GroupFifo &fifo, const BuildOut &o, hipStream_t s);
 int launch_bt_usages(const uint32_t *group_parent, uint32_t n_groups, uint32_t dense_groups, uint32_t tick,

@@ -121,3 +121,43 @@ def test_frame_on_device_built_tree(svo, orc, gpu):
         assert m is None, f"{key}: {m}"
     assert np.array_equal(gpu.ReadBuffer(N.Usages, svo.USAGE_DTYPE), usages)
     assert int(gpu.ReadBuffer(N.ChildRequestID, np.uint32, 1)[0]) == ref["request_count"]
+
+
+def _cubes_scene(svo):
+    import ctypes as C
+    shapes = (C.c_float * 12)(0.1, 0.2, 0.15, 0.45, 0.6, 0.4, 0.5, 0.05, 0.3, 0.93, 0.41, 0.77)
+    g = svo.Geometry()
+    g.kind = svo.GEOMETRY_CUBES
+    g.count = 2
+    g.shapes = C.cast(shapes, C.POINTER(C.c_float))
+    return g, shapes
+
+
+@pytest.mark.parametrize("case", ["f0_spheres", "config1_noise", "noise_d9_b3", "cubes", "heightfield"])
+def test_device_build_every_geometry(svo, gpu, case):
+    """svo_tree_build_device == svo_tree_build for spheres, the sphere + noise SDF, boxes and a
+    heightfield given as a geometry."""
+    keep = None
+    if case == "f0_spheres":
+        spec = svo.scenes.f0_scene(8)
+        geom, B, D, size = spec.geometry, spec.base_depth, spec.max_depth, spec.max_size
+    elif case == "config1_noise":
+        spec = svo.scenes.config1_scene()
+        geom, B, D, size = spec.geometry, spec.base_depth, spec.max_depth, spec.max_size
+    elif case == "noise_d9_b3":
+        spec = svo.scenes.sphere_noise_scene(3, 9, base_depth=3)
+        geom, B, D, size = spec.geometry, spec.base_depth, spec.max_depth, spec.max_size
+    elif case == "cubes":
+        geom, keep = _cubes_scene(svo)
+        B, D, size = 3, 8, BIG
+    else:
+        geom, B, D, size = terrain_geometry(svo, 256), 3, 9, BIG
+    host = svo.build_tree(geom, B, D, size)
+    count, dev = gpu.BuildTreeDevice(geom, B, D, size, with_usages=True, usage_tick=2, host_copy=True)
+    assert count == host.block_count == dev.block_count
+    assert np.array_equal(dev.bases, host.bases)
+    m = first_mismatch(dev.blocks, host.blocks)
+    assert m is None, f"blocks: {m}"
+    assert np.array_equal(gpu.ReadBuffer(svo.BufferName.Usages, svo.USAGE_DTYPE), host.initial_usages(2))
+    assert host.allocated_groups > 0
+    del keep
