@@ -176,8 +176,8 @@ def adjoint_leg(svo, steps: int, fit_steps: int) -> dict:
     """Forward + adjoint on the depth-10 sphere+noise scene (seed 3), 32 views of 512x512:
     targets rendered from the true leaf colours and opacities, parameters perturbed by
     N(0, 0.1) (colour) and N(0, 0.05) (opacity), clipped to [0, 1]; L2 loss.  Times the
-    plain forward of the same views, the compositing forward and forward + adjoint, then
-    runs the Adam fit."""
+    plain forward of the same views, the compositing forward and the adjoint launch (which
+    is forward + adjoint in one traversal), then runs the Adam fit."""
     B = svo.BufferName
     spec = svo.scenes.config5_scene()
     tree = spec.build()
@@ -233,17 +233,20 @@ def adjoint_leg(svo, steps: int, fit_steps: int) -> dict:
     finally:
         cm.Dispose()
     rate = lambda ms: round(n / (ms / 1e3) / 1e6, 2)
+    # one adjoint launch is a whole forward + adjoint step: it composites the image, layers
+    # and loss in the same traversal that back-propagates (the Adam loop runs exactly that)
     return {
-        "value": rate(ms_fwd + ms_adj),
+        "value": rate(ms_adj),
         "unit": "Mcones/s",
-        "ms_per_step": round(ms_fwd + ms_adj, 3),
+        "ms_per_step": round(ms_adj, 3),
         "workload": "depth-10 sphere+noise (seed 3), 32 views of 512x512, 4 layers, L2 loss",
         "cones_per_step": n,
         "plain_forward_ms": round(ms_plain, 3),
         "composite_forward_ms": round(ms_fwd, 3),
-        "forward_plus_adjoint_ms": round(ms_fwd + ms_adj, 3),
-        "adjoint_pass_ms": round(ms_adj, 3),
-        "cost_vs_plain_forward": round((ms_fwd + ms_adj) / ms_plain, 3),
+        "forward_plus_adjoint_ms": round(ms_adj, 3),
+        "cost_vs_plain_forward": round(ms_adj / ms_plain, 3),
+        "separate_forward_then_adjoint_ms": round(ms_fwd + ms_adj, 3),
+        "separate_cost_vs_plain_forward": round((ms_fwd + ms_adj) / ms_plain, 3),
         "adam_fit": {"steps": fit_steps, "loss_first": round(losses[0], 4), "loss_last": round(final, 4),
                      "seconds": round(fit_s, 2)},
     }

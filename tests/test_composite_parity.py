@@ -89,6 +89,29 @@ def test_opaque_composite_equals_plain_frame(svo, orc, gpu, setup):
     assert np.max(np.abs(got - ref["rgba"])) <= 1e-6
 
 
+def test_adjoint_pass_is_also_the_forward(svo, orc, gpu, setup):
+    """One adjoint launch computes the forward composite in the same traversal: its image and
+    layers equal the forward launch's bit for bit and its loss to the rounding of a sum (an
+    optimisation step is one launch)."""
+    tree, bases, blocks, views = setup
+    n = sum(v.screen_size[0] * v.screen_size[1] for v in views)
+    truth = _params_from_blocks(blocks)
+    params = _perturbed(truth, 3)
+    layers, bg = orc.composite_layers(views, bases, blocks, K, threads=8)
+    _, target, _ = orc.composite_loss(layers, bg, truth.astype(np.float64), want_image=True)
+    _bind(svo, gpu, bases, blocks, params, n, target=target.astype(np.float32))
+    B = svo.BufferName
+    out = []
+    for adjoint in (False, True):
+        loss = gpu.Composite(views, max_layers=K, adjoint=adjoint, layers=True)
+        gpu.Wait()
+        out.append((loss, gpu.ReadBuffer(B.OutputImage, np.float32, 4 * n), gpu.ReadBuffer(B.Layers, np.uint32, n * K)))
+    assert abs(out[0][0] - out[1][0]) <= 1e-12 * abs(out[0][0])  # a double sum; the order of its atomics varies
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+    assert np.abs(gpu.ReadBuffer(B.LeafGradients, np.float32)).sum() > 0
+
+
 def test_adjoint_matches_fp64_and_finite_differences(svo, orc, gpu, setup):
     tree, bases, blocks, views = setup
     n = sum(v.screen_size[0] * v.screen_size[1] for v in views)
