@@ -252,6 +252,113 @@ def adjoint_leg(svo, steps: int, fit_steps: int) -> dict:
     }
 
 
+def wavefront_leg(svo, tree, cams, frames: int = 3) -> dict:
+    """The reference's own frame structure on the same GPU: MainWindow.RunWavefront's loop
+    (init, then runBaseTrace / runBlockTrace launches with a blocking queue-size read-back
+    between them, evaluateBackground, resolveRemainders, drawTrace; MainWindow.cs:356-481)
+    driven through svo_enqueue, as an unchanged caller would.  Times whole frames; checks
+    the image against the fused kernel's."""
+    K, B = svo.KernelName, svo.BufferName
+    cm = svo.ComputeManager(0)
+    try:
+        cm.LoadTree(tree, with_usages=True, usage_tick=0)
+        W, H = cams[0].screen_size[0], cams[0].screen_size[1]
+        n = W * H
+        cm.InitBuffer(B.ChildRequestID, np.zeros(1, np.uint32))
+        cm.InitDeviceBuffer(B.ChildRequests, n, 40)
+        for name, size in ((B.Origins, 16), (B.Directions, 16), (B.FoVs, 4), (B.Locations, 32), (B.Depths, 1),
+                           (B.Weightings, 1), (B.ParentTraces, 4), (B.ColourRs, 1), (B.ColourGs, 1),
+                           (B.ColourBs, 1), (B.BaseTraces, 4), (B.BaseTraceQueue, 4), (B.BlockTraces, 4),
+                           (B.BlockTraceQueue, 4), (B.BackgroundQueue, 4), (B.MaterialQueue, 4),
+                           (B.FinalColourRs, 4), (B.FinalColourGs, 4), (B.FinalColourBs, 4),
+                           (B.FinalWeightings, 4), (B.OutputImage, 16)):
+            cm.InitDeviceBuffer(name, n, size)
+        for name in (B.BaseTraceQueueID, B.BlockTraceQueueID, B.BackgroundQueueID, B.MaterialQueueID,
+                     B.AccumulatorID):
+            cm.InitBuffer(name, np.zeros(1, np.uint32))
+
+        def bind(kernel, names):
+            for param in names:
+                cm.SetArg(kernel, param, B[param])
+
+        def frame(cam):
+            for name in (B.BlockTraceQueueID, B.BaseTraceQueueID, B.BackgroundQueueID, B.MaterialQueueID,
+                         B.AccumulatorID, B.ChildRequestID):
+                cm.ZeroIDBuffer(name)
+            cam.max_child_request_id = n
+            cm.SetArg(K.Init, "input", cam)
+            bind(K.Init, ["Origins", "Directions", "FoVs", "Locations", "Depths", "BaseTraceQueue",
+                          "BaseTraceQueueID", "BackgroundQueue", "BackgroundQueueID", "ColourRs", "ColourGs",
+                          "ColourBs", "Weightings", "ParentTraces", "FinalColourRs", "FinalColourGs",
+                          "FinalColourBs", "FinalWeightings"])
+            cm.Enqueue(K.Init, [W, H])
+            cm.SetArg(K.RunBaseTrace, "input", cam)
+            cm.SetArg(K.RunBlockTrace, "input", cam)
+            iterations = 0
+            while True:
+                base_size = cm.ResetIDBuffer(B.BaseTraceQueueID)
+                if base_size > 0:
+                    cm.Swap(B.BaseTraces, B.BaseTraceQueue)
+                    bind(K.RunBaseTrace, ["Bases", "Origins", "Directions", "FoVs", "Locations", "Weightings",
+                                          "Depths", "BaseTraces", "BaseTraceQueue", "BaseTraceQueueID",
+                                          "BlockTraceQueue", "BlockTraceQueueID", "BackgroundQueue",
+                                          "BackgroundQueueID"])
+                    cm.Enqueue(K.RunBaseTrace, [base_size])
+                block_size = cm.ResetIDBuffer(B.BlockTraceQueueID)
+                if block_size > 0:
+                    cm.Swap(B.BlockTraces, B.BlockTraceQueue)
+                    bind(K.RunBlockTrace, ["Blocks", "Usages", "ChildRequestID", "ChildRequests", "Origins",
+                                           "Directions", "FoVs", "Locations", "Weightings", "Depths",
+                                           "BlockTraces", "BlockTraceQueue", "BlockTraceQueueID", "BaseTraceQueue",
+                                           "BaseTraceQueueID", "BackgroundQueue", "BackgroundQueueID",
+                                           "MaterialQueue", "MaterialQueueID", "ParentTraces"])
+                    cm.Enqueue(K.RunBlockTrace, [block_size])
+                if base_size == 0 and block_size == 0:
+                    break
+                iterations += 1
+            bg = cm.ResetIDBuffer(B.BackgroundQueueID)
+            cm.SetArg(K.EvaluateBackground, "input", cam)
+            if bg:
+                bind(K.EvaluateBackground, ["BackgroundQueue", "Directions", "ParentTraces", "ColourRs",
+                                            "ColourGs", "ColourBs", "Weightings", "FinalColourRs", "FinalColourGs",
+                                            "FinalColourBs", "FinalWeightings"])
+                cm.Enqueue(K.EvaluateBackground, [bg])
+            cm.SetArg(K.ResolveRemainders, "input", cam)
+            bind(K.ResolveRemainders, ["Blocks", "Usages", "Locations", "Depths", "MaterialQueue", "FinalColourRs",
+                                       "FinalColourGs", "FinalColourBs", "FinalWeightings", "Weightings",
+                                       "ParentTraces"])
+            mat = cm.ResetIDBuffer(B.MaterialQueueID)
+            if mat:
+                cm.Enqueue(K.ResolveRemainders, [mat])
+            bind(K.DrawTrace, ["FinalColourRs", "FinalColourGs", "FinalColourBs", "FinalWeightings"])
+            cm.SetArg(K.DrawTrace, "outputImage", B.OutputImage)
+            cm.SetArg(K.DrawTrace, "input", cam)
+            cm.Enqueue(K.DrawTrace, [W, H])
+            cm.Wait()
+            return iterations
+
+        frame(cams[0])  # warm-up
+        ms, its = [], []
+        for cam in cams[1:1 + frames]:
+            t0 = time.perf_counter()
+            its.append(frame(cam))
+            ms.append((time.perf_counter() - t0) * 1e3)
+        img = cm.ReadBuffer(B.OutputImage, np.float32, 4 * n)
+        # the fused kernel on the same camera and tree, for the image check
+        cm.InitDeviceBuffer(B.HitIndices, n, 4)
+        cm.InitDeviceBuffer(B.VisitCounts, n, 4)
+        cm.TraceFrame(cams[frames])
+        cm.Wait()
+        same = np.array_equal(img, cm.ReadBuffer(B.OutputImage, np.float32, 4 * n))
+    finally:
+        cm.Dispose()
+    t = float(np.mean(ms))
+    return {"ms_per_frame": round(t, 2), "value": round(n / (t / 1e3) / 1e6, 2), "unit": "Mcones/s",
+            "queue_iterations_per_frame": float(np.mean(its)), "image_equals_fused_kernel": bool(same),
+            "what": "the reference's RunWavefront loop through svo_enqueue (one launch per kernel per "
+                    "iteration, blocking queue-size read-backs), same frames and tree"}
+
+
 def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
     """Streaming: a 3840x2160 camera flies the seeded 300-frame spline over the fBm terrain
     generated on the device (16384^2 grid) and refined on demand to depth 14 in a fixed pool of
@@ -352,6 +459,7 @@ def main() -> int:
     ap.add_argument("--waves", type=int, default=0, help="register budget of the trace kernel (0 = default, or 6)")
     ap.add_argument("--chunk", type=int, default=0, help="cones a wave claims per atomic")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched leg")
+    ap.add_argument("--no-wavefront", action="store_true", help="skip the kernel-by-kernel reference-loop leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming leg")
     ap.add_argument("--stream-frames", type=int, default=300)
@@ -500,6 +608,12 @@ def main() -> int:
                         "what": "2 GiB device-to-device copy, read + write bytes",
                         "achieved_vs_copy": round(achieved / copy_gbs, 4)}
     log(f"device copy: {copy_gbs:.0f} GB/s; trace kernel at {achieved / copy_gbs:.3f} of it")
+    if not args.no_wavefront:
+        log("reference loop (kernel by kernel) leg...")
+        line["kernel_by_kernel"] = wavefront_leg(svo, tree, cams)
+        log(f"kernel-by-kernel: {line['kernel_by_kernel']['ms_per_frame']} ms/frame "
+            f"({line['kernel_by_kernel']['queue_iterations_per_frame']:.0f} queue iterations), image equal: "
+            f"{line['kernel_by_kernel']['image_equals_fused_kernel']}")
     if not args.no_batched:
         log("batched 64-camera leg...")
         line["batched"] = batched_leg(svo, cm, max(3, min(K, 10)), 2, usages0, opts)
