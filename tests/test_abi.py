@@ -75,6 +75,8 @@ def test_errors_raise_with_method_and_code(svo):
     assert lib.svo_stream_seed(None, None, 4) == -1
     assert lib.svo_stream_update(None, None, None, None) == -1
     assert lib.svo_terrain_destroy(None) == 0
+    assert lib.svo_terrain_build_tree(None, None, 4, 8, 0, 0, 0, None, None) == -1
+    assert lib.svo_tree_build_device(None, None, 4, 8, 0, 0, 0, None, None) == -1
 
 
 def test_create_without_device_fails_cleanly(svo):
