@@ -515,6 +515,11 @@ int svo_tree_free(svo_tree *tree);
 /* Uploads Bases, Blocks and (optionally) initial Usages of a tree into the named buffers. */
 int svo_context_load_tree(svo_context *ctx, const svo_tree *tree, int with_usages, uint16_t usage_tick);
 
+/* No reference counterpart; a self-test hook: the device's floatToUlong (kind 0: n floats in,
+ * n u64 out) or ulongToFloat (kind 1: n u64 in, n floats out), wavefront.cl:200-209, applied
+ * to host arrays, so the device conversions can be compared with any host restatement. */
+int svo_convert_probe(svo_context *ctx, int kind, const void *in, void *out, size_t n);
+
 /* Separable camera table used for ray generation (replaces native_sin/native_cos of
  * setupInitialTrace, wavefront.cl:563-595): cos_u/sin_u per column, cos_v/sin_v per row. */
 int svo_camera_lut(const svo_trace_input *input, float *cos_u, float *sin_u, float *cos_v, float *sin_v);

@@ -82,6 +82,10 @@ def lib() -> C.CDLL:
         L.orc_float_to_ulong.restype = C.c_uint64
         L.orc_ulong_to_float.argtypes = [C.c_uint64]
         L.orc_ulong_to_float.restype = C.c_float
+        L.orc_float_to_ulong_n.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.orc_float_to_ulong_n.restype = None
+        L.orc_ulong_to_float_n.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.orc_ulong_to_float_n.restype = None
         L.orc_round_ulong.argtypes = [C.c_uint64, C.c_uint32, C.c_int]
         L.orc_round_ulong.restype = C.c_uint64
         L.orc_pow_sum.argtypes = [C.c_uint32]
@@ -150,6 +154,20 @@ def _u64x3(v):
 
 def float_to_ulong(x: float) -> int:
     return int(lib().orc_float_to_ulong(x))
+
+
+def float_to_ulong_n(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.size, np.uint64)
+    lib().orc_float_to_ulong_n(x.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def ulong_to_float_n(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    out = np.empty(x.size, np.float32)
+    lib().orc_ulong_to_float_n(x.ctypes.data, out.ctypes.data, x.size)
+    return out
 
 
 def ulong_to_float(x: int) -> float:

@@ -49,6 +49,14 @@ uint64_t orc_float_to_ulong(float x) {
 /* ulongToFloat, wavefront.cl:209: (float)x / 2^64, exact scaling.  Pinned as above. */
 float orc_ulong_to_float(uint64_t x) { return (float)x * 5.42101086242752217e-20f; }
 
+/* Array forms of the two conversions (for the exhaustive tests). */
+void orc_float_to_ulong_n(const float *x, uint64_t *out, size_t n) {
+  for (size_t i = 0; i < n; i++) out[i] = orc_float_to_ulong(x[i]);
+}
+void orc_ulong_to_float_n(const uint64_t *x, float *out, size_t n) {
+  for (size_t i = 0; i < n; i++) out[i] = orc_ulong_to_float(x[i]);
+}
+
 /* roundUlong, wavefront.cl:211-221: distance to the next (round_up) or just past the
  * previous level-(depth+1) cell boundary.  Pinned: boundary-landing property test. */
 uint64_t orc_round_ulong(uint64_t value, uint32_t depth, int round_up) {

@@ -91,6 +91,8 @@ typedef struct {
 /* ---- individual helpers (exported for the pinning tests) ---- */
 uint64_t orc_float_to_ulong(float x);
 float orc_ulong_to_float(uint64_t x);
+void orc_float_to_ulong_n(const float *x, uint64_t *out, size_t n);
+void orc_ulong_to_float_n(const uint64_t *x, float *out, size_t n);
 uint64_t orc_round_ulong(uint64_t value, uint32_t depth, int round_up);
 uint32_t orc_pow_sum(uint32_t depth);
 uint32_t orc_chunk_position(uint32_t depth, const uint64_t loc[3]);

@@ -400,6 +400,7 @@ _PROTOS = {
     "svo_stream_seed": [C.c_void_p, C.c_void_p, C.c_uint32],
     "svo_terrain_build_tree": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint16,
                                C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)],
+    "svo_convert_probe": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t],
     "svo_tree_build_device": [C.c_void_p, C.POINTER(Geometry), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint16,
                               C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)],
     "svo_stream_update": [C.c_void_p, C.c_void_p, C.POINTER(StreamOptions), C.POINTER(StreamStats)],
@@ -657,6 +658,14 @@ class ComputeManager:
                                            1 if with_usages else 0, usage_tick,
                                            C.byref(h) if host_copy else None, C.byref(count)), "BuildTreeDevice")
         return int(count.value), (Octree(h) if host_copy else None)
+
+    def ConvertProbe(self, kind: int, values: np.ndarray) -> np.ndarray:
+        """svo_convert_probe: the device's floatToUlong (kind 0, float32 in) or ulongToFloat
+        (kind 1, uint64 in) over an array."""
+        x = np.ascontiguousarray(values, dtype=np.float32 if kind == 0 else np.uint64)
+        out = np.empty(x.size, np.uint64 if kind == 0 else np.float32)
+        _check(lib().svo_convert_probe(self._h, kind, x.ctypes.data, out.ctypes.data, x.size), "ConvertProbe")
+        return out
 
     def StreamSeed(self, terrain: "Terrain", base_depth: int) -> None:
         """svo_stream_seed: Bases, the dense block level and the usage table of a fresh pool."""

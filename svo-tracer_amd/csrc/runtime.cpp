@@ -418,6 +418,25 @@ int svo_frame_stats_get(svo_context *ctx, svo_frame_stats *stats) {
   return SVO_OK;
 }
 
+int svo_convert_probe(svo_context *ctx, int kind, const void *in, void *out, size_t n) {
+  CHECK_CTX(ctx, "svo_convert_probe");
+  if ((n && (!in || !out)) || (kind != 0 && kind != 1))
+    return fail(SVO_ERR_INVALID_ARGUMENT, "svo_convert_probe: bad argument");
+  if (!n) return SVO_OK;
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  const size_t in_bytes = n * (kind == 0 ? 4 : 8), out_bytes = n * (kind == 0 ? 8 : 4);
+  std::shared_ptr<DeviceBuffer> a, b;
+  int rc = new_buffer(in_bytes, &a);
+  if (!rc) rc = new_buffer(out_bytes, &b);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(a->ptr, in, in_bytes, hipMemcpyHostToDevice, ctx->stream), "svo_convert_probe: upload");
+  if (launch_convert((uint32_t)kind, a->ptr, b->ptr, n, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "svo_convert_probe: launch");
+  HIP_TRY(hipMemcpyAsync(out, b->ptr, out_bytes, hipMemcpyDeviceToHost, ctx->stream), "svo_convert_probe: download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "svo_convert_probe: sync");
+  return SVO_OK;
+}
+
 int svo_context_load_tree(svo_context *ctx, const svo_tree *tree, int with_usages, uint16_t usage_tick) {
   CHECK_CTX(ctx, "svo_context_load_tree");
   if (!tree) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_context_load_tree: null tree");

@@ -96,6 +96,8 @@ int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu);
 int launch_tile_order(const uint32_t *tile_cost, uint32_t *order, const uint32_t range_begin[9], hipStream_t stream);
 // Sets n words to zero (a kernel, so it may be captured in a graph).
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t stream);
+// kind 0: floatToUlong of n floats into n u64; kind 1: ulongToFloat of n u64 into n floats.
+int launch_convert(uint32_t kind, const void *in, void *out, size_t n, hipStream_t stream);
 int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream);
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream);
 // Compositing modes of the trace kernel (camera table, 256-lane work-groups).

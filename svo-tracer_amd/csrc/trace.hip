@@ -583,6 +583,17 @@ __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
+// The device's floatToUlong / ulongToFloat (wavefront.cl:200-209) applied to an array, for
+// comparing the device conversions with the host's over large samples.
+__global__ void convert_kernel(uint32_t kind, const void *in, void *out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (kind == 0)
+      static_cast<uint64_t *>(out)[i] = float_to_ulong(static_cast<const float *>(in)[i]);
+    else
+      static_cast<float *>(out)[i] = ulong_to_float(static_cast<const uint64_t *>(in)[i]);
+  }
+}
+
 __global__ void frame_prepare_kernel(uint32_t *counters, unsigned long long *totals) {
   const uint32_t i = threadIdx.x;
   if (i < 8u) counters[i * kCounterStride] = 0u;
@@ -780,6 +791,13 @@ int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t stream) {
   if (n == 0) return 0;
   const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 1024u);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_convert(uint32_t kind, const void *in, void *out, size_t n, hipStream_t stream) {
+  if (n == 0) return 0;
+  const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(convert_kernel, dim3(blocks), dim3(256), 0, stream, kind, in, out, n);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
