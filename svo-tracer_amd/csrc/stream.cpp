@@ -598,9 +598,9 @@ int build_tree_device(Context *ctx, const GeomView &g, uint32_t base_depth, uint
     // records among the DFS writes: a is one when every written address above it was
     // written later (post(a) < post(b) for all written b > a); all addresses above M1 are
     // DFS writes, so positions compare within one phase
-    uint32_t window = 4096;
+    uint64_t window = 4096;
     for (;;) {
-      const uint32_t lo = (M - M1 > window) ? M - window + 1 : M1 + 1;
+      const uint32_t lo = ((uint64_t)(M - M1) > window) ? (uint32_t)(M - window + 1) : M1 + 1;
       std::vector<uint32_t> pw(M - lo + 1);
       if ((rc = bd.read(bd.ptr<uint32_t>(post) + lo, pw.data(), pw.size() * 4))) return rc;
       uint32_t suf = pw.back(), a = M;  // a: lowest record of the run so far
