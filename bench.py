@@ -359,14 +359,14 @@ def wavefront_leg(svo, tree, cams, frames: int = 3) -> dict:
                     "iteration, blocking queue-size read-backs), same frames and tree"}
 
 
-def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
+def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blocks: int = 26) -> dict:
     """Streaming: a 3840x2160 camera flies the seeded 300-frame spline over the fBm terrain
     generated on the device (16384^2 grid) and refined on demand to depth 14 in a fixed pool of
     2^26 blocks; per frame the trace makes child requests and ticks usages, and the streaming
     step grafts up to 65,536 bricks, collects groups unused for 30 frames and compacts when
     fewer than 10 % of the groups are free."""
     B = svo.BufferName
-    Bd, D, n_groups = 4, 14, 1 << 23
+    Bd, D, n_groups = 4, 14, 1 << (log2_blocks - 3)
     n = width * height
     cm = svo.ComputeManager(0)
     try:
@@ -424,7 +424,7 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
         "unit": "Mcones/s",
         "ms_per_step": round(frame_ms, 3),
         "workload": f"{width}x{height}, {frames}-frame seeded spline (seed 3), fBm terrain 16384^2 generated on "
-                    f"the device, refined on demand to depth {D}, pool 2^26 blocks, <= 65536 bricks/frame, "
+                    f"the device, refined on demand to depth {D}, pool 2^{log2_blocks} blocks, <= 65536 bricks/frame, "
                     "prune age 30 frames, GC below 10 % free",
         "frames": frames,
         "trace_ms_mean": round(float(np.mean(trace_ms)), 3),
@@ -433,6 +433,7 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160) -> dict:
         "bricks_total": int(sum(s["bricks"] for s in stats)),
         "bricks_per_frame_max": int(max(s["bricks"] for s in stats)),
         "requests_per_frame_mean": round(float(np.mean([s["requests"] for s in stats])), 1),
+        "collected_total": int(sum(s["collected"] for s in stats)),
         "gc_runs": int(sum(s["gc_runs"] for s in stats)),
         "reclaimed_total": int(sum(s["reclaimed"] for s in stats)),
         "moved_total": int(sum(s["moved"] for s in stats)),
@@ -627,6 +628,12 @@ def main() -> int:
         log("streaming leg...")
         line["streaming"] = stream_leg(svo, args.stream_frames)
         log(f"streaming: {line['streaming']['value']} Mcones/s, update {line['streaming']['update_ms_mean']} ms/frame")
+        log("streaming leg with a pool of 2^22 blocks (garbage collection under pressure)...")
+        line["streaming_small_pool"] = stream_leg(svo, 150, log2_blocks=22)
+        sp = line["streaming_small_pool"]
+        log(f"streaming, small pool: {sp['value']} Mcones/s, {sp['collected_total']} stale groups collected by the "
+            f"grafts, {sp['gc_runs']} garbage collections ({sp['reclaimed_total']} reclaimed, {sp['moved_total']} "
+            f"moved), update {sp['update_ms_mean']} ms/frame")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample)...")
         line["cpu_baseline"] = cpu_baseline(svo, tree, cams, args.cpu_seconds)
