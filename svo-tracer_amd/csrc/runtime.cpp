@@ -170,6 +170,7 @@ int svo_destroy(svo_context *ctx) {
   for (auto &b : ctx->tree_scratch) b.reset();
   if (ctx->tree_host) (void)hipHostFree(ctx->tree_host);
   if (ctx->stream_host) (void)hipHostFree(ctx->stream_host);
+  if (ctx->id_host) (void)hipHostFree(ctx->id_host);
   if (ctx->cam_copied) (void)hipEventSynchronize(ctx->cam_copied);
   if (ctx->cam_host) (void)hipHostFree(ctx->cam_host);
   if (ctx->cam_copied) (void)hipEventDestroy(ctx->cam_copied);
@@ -283,12 +284,12 @@ int svo_reset_id(svo_context *ctx, int name, uint32_t *count_out) {
   std::shared_ptr<DeviceBuffer> b;
   int rc = checked_range(ctx, name, 0, 4, "svo_reset_id", &b);
   if (rc) return rc;
-  uint32_t v = 0;
-  HIP_TRY(hipMemcpyAsync(&v, b->ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "svo_reset_id: read");
-  const uint32_t zero = 0;
-  HIP_TRY(hipMemcpyAsync(b->ptr, &zero, 4, hipMemcpyHostToDevice, ctx->stream), "svo_reset_id: write");
+  // read into a pinned word and clear on the device: one round trip per call
+  if (!ctx->id_host) HIP_TRY(hipHostMalloc((void **)&ctx->id_host, 4, hipHostMallocDefault), "hipHostMalloc");
+  HIP_TRY(hipMemcpyAsync(ctx->id_host, b->ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "svo_reset_id: read");
+  HIP_TRY(hipMemsetAsync(b->ptr, 0, 4, ctx->stream), "svo_reset_id: clear");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "svo_reset_id: sync");
-  if (count_out) *count_out = v;
+  if (count_out) *count_out = *ctx->id_host;
   return SVO_OK;
 }
 
@@ -297,8 +298,7 @@ int svo_zero_id(svo_context *ctx, int name) {
   std::shared_ptr<DeviceBuffer> b;
   int rc = checked_range(ctx, name, 0, 4, "svo_zero_id", &b);
   if (rc) return rc;
-  const uint32_t zero = 0;
-  HIP_TRY(hipMemcpyAsync(b->ptr, &zero, 4, hipMemcpyHostToDevice, ctx->stream), "svo_zero_id");
+  HIP_TRY(hipMemsetAsync(b->ptr, 0, 4, ctx->stream), "svo_zero_id");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "svo_zero_id: sync");
   return SVO_OK;
 }

@@ -113,6 +113,7 @@ struct Context {
   // scratch of evaluateMaterial: per-entry counts and first ids, chunk sums, queue flags
   std::shared_ptr<DeviceBuffer> material_scratch[5];
   uint32_t *stream_host = nullptr;
+  uint32_t *id_host = nullptr;  // pinned word for the queue-counter read-backs (svo_reset_id)
   // batched cameras: concatenated camera tables (cache key), camera records
   std::shared_ptr<DeviceBuffer> scratch_batch_lut;
   std::vector<float> batch_lut_key;
