@@ -457,7 +457,7 @@ def main() -> int:
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--waves", type=int, default=0, help="register budget of the trace kernel (0 = default, or 6)")
+    ap.add_argument("--waves", type=int, default=0, help="register budget of the trace kernel (0 = default 5, or 4..6)")
     ap.add_argument("--chunk", type=int, default=0, help="cones a wave claims per atomic")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-camera batched leg")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the kernel-by-kernel reference-loop leg")

@@ -294,7 +294,7 @@ typedef struct {
   uint32_t refill_lanes;   /* lanes idle before a wave fetches new cones, 0 = 64 (whole wave) */
   uint32_t stack_levels;   /* LDS short-stack levels, 0 = from max_tree_depth */
   uint32_t work_chunk;     /* cones a wave claims per atomic (rounded up to 64), 0 = 64 (one 8x8 tile) */
-  uint32_t waves_per_simd; /* register budget of the trace kernel: 0 (default) or 6 waves per SIMD */
+  uint32_t waves_per_simd; /* register budget of the trace kernel: 4, 5 or 6 waves per SIMD; 0 = 5 */
   uint32_t reserved[1];
 } svo_frame_options;
 

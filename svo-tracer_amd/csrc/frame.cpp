@@ -100,9 +100,10 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, 
   while (levels > 1 && bases_bytes + (size_t)levels * wg * 4 > lds_limit) levels--;
   size_t lds = bases_bytes + (size_t)levels * wg * 4;
   if (lds > lds_limit) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup too large for LDS");
-  const int waves = o.waves_per_simd ? (int)o.waves_per_simd : 6;
-  if (waves != 6)
-    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: waves_per_simd must be 0 (default) or 6");
+  // register budget: 5 waves per SIMD (96 VGPRs) keeps the step loop free of spills
+  const int waves = o.waves_per_simd ? (int)o.waves_per_simd : 5;
+  if (waves < 4 || waves > 6)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: waves_per_simd must be 0 (default, 5), 4, 5 or 6");
   int per_cu = 0;
   if (trace_occupancy(wg, waves, lds, &per_cu) != 0 || per_cu <= 0)
     return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");

@@ -77,10 +77,12 @@ trace_frame_kernel(const FrameParams p) {
   uint32_t range_step = 0;  // wave-uniform: ranges tried, in order xcd, xcd+1, ...
   bool fetching = true;     // wave-uniform
   uint32_t cur = 0, cur_end = 0, cur_t0 = 0;  // wave-uniform: the wave's claimed slice of work
-  // wave-uniform: the tile the wave took at its last refill and the loop iterations since
+  // the tile the wave took at its last refill (wave-uniform) and the steps each lane ran since
   uint32_t cur_tile = kU32Max, age = 0;
   auto record_cost = [&]() {
-    if (p.tile_cost && cur_tile != kU32Max && lane == 0u) p.tile_cost[cur_tile] = age;
+    uint32_t a = age;  // iterations a lane ran on the tile; the tile took the longest of them
+    for (int off = 32; off > 0; off >>= 1) a = max(a, (uint32_t)__shfl_xor((int)a, off));
+    if (p.tile_cost && cur_tile != kU32Max && lane == 0u) p.tile_cost[cur_tile] = a;
     cur_tile = kU32Max;
     age = 0;
   };
@@ -358,8 +360,10 @@ trace_frame_kernel(const FrameParams p) {
       if (!fetching) break;
       continue;
     }
-    age++;
-    if (active) {
+    // Step the running lanes until enough lanes are idle to refill: a lane leaves this loop
+    // when its cone ends, so the exec mask keeps finished lanes' registers untouched.
+    while (active) {
+      age++;
       // ---- one step of the reference state machine ----
       // A step that moves to another block only sets node_addr and need_load; the block is
       // loaded once, at the end of the step.
@@ -478,6 +482,8 @@ trace_frame_kernel(const FrameParams p) {
         }
       }
       if (need_load) node = load_block(p.blocks, p.n_blocks, node_addr);
+      const uint32_t running = (uint32_t)__popcll(__ballot(active));
+      if (fetching && 64u - running >= p.refill) break;
     }
   }
 
@@ -775,6 +781,8 @@ int occupancy_waves(int wg, size_t lds, int *blocks_per_cu) {
 
 int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
   switch (waves) {
+    case 4: return occupancy_waves<4>(wg, lds_bytes, blocks_per_cu);
+    case 5: return occupancy_waves<5>(wg, lds_bytes, blocks_per_cu);
     case 6: return occupancy_waves<6>(wg, lds_bytes, blocks_per_cu);
     default: return -1;
   }
@@ -808,6 +816,8 @@ int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStre
 
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream) {
   switch (shape.waves) {
+    case 4: return launch_waves<4>(p, shape, stream);
+    case 5: return launch_waves<5>(p, shape, stream);
     case 6: return launch_waves<6>(p, shape, stream);
     default: return -1;
   }
