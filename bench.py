@@ -433,7 +433,7 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blo
         "bricks_total": int(sum(s["bricks"] for s in stats)),
         "bricks_per_frame_max": int(max(s["bricks"] for s in stats)),
         "requests_per_frame_mean": round(float(np.mean([s["requests"] for s in stats])), 1),
-        "collected_total": int(sum(s["collected"] for s in stats)),
+        "collected_total": int(sum(s["collected"] for s in stats)),  # groups the grafts' collection gathered
         "gc_runs": int(sum(s["gc_runs"] for s in stats)),
         "reclaimed_total": int(sum(s["reclaimed"] for s in stats)),
         "moved_total": int(sum(s["moved"] for s in stats)),
@@ -631,9 +631,9 @@ def main() -> int:
         log("streaming leg with a pool of 2^22 blocks (garbage collection under pressure)...")
         line["streaming_small_pool"] = stream_leg(svo, 150, log2_blocks=22)
         sp = line["streaming_small_pool"]
-        log(f"streaming, small pool: {sp['value']} Mcones/s, {sp['collected_total']} stale groups collected by the "
-            f"grafts, {sp['gc_runs']} garbage collections ({sp['reclaimed_total']} reclaimed, {sp['moved_total']} "
-            f"moved), update {sp['update_ms_mean']} ms/frame")
+        log(f"streaming, small pool: {sp['value']} Mcones/s, {sp['collected_total']} groups gathered by the grafts' "
+            f"collection (free, or unused for 30 frames), {sp['gc_runs']} garbage collections "
+            f"({sp['reclaimed_total']} reclaimed, {sp['moved_total']} moved), update {sp['update_ms_mean']} ms/frame")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on a bounded sample)...")
         line["cpu_baseline"] = cpu_baseline(svo, tree, cams, args.cpu_seconds)
