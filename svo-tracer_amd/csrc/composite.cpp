@@ -93,12 +93,15 @@ extern "C" int svo_composite(svo_context *ctx, const svo_trace_input *views, con
     return fail(SVO_ERR_HIP, "svo_composite: occupancy query failed");
   plan.shape.grid = std::max(8, (per_cu * std::max(1, ctx->cu_count) / 8) * 8);
 
-  HIP_TRY(hipMemcpyAsync((void *)p.cams, ctx->cam_host, n_views * sizeof(CamRec), hipMemcpyHostToDevice, ctx->stream),
-          "camera upload");
-  HIP_TRY(hipEventRecord(ctx->cam_copied, ctx->stream), "hipEventRecord");
-  if (launch_frame_prepare(p.counters, p.totals, ctx->stream) != 0)
-    return hip_fail(hipGetLastError(), "svo_composite: prepare launch");
-  if (launch_schedule(plan, ctx->stream) != 0) return hip_fail(hipGetLastError(), "svo_composite: tile order launch");
+  const uint32_t n_inline = n_views <= kInlineCams ? n_views : 0u;
+  if (!n_inline) {
+    HIP_TRY(hipMemcpyAsync((void *)p.cams, ctx->cam_host, n_views * sizeof(CamRec), hipMemcpyHostToDevice,
+                           ctx->stream),
+            "camera upload");
+    HIP_TRY(hipEventRecord(ctx->cam_copied, ctx->stream), "hipEventRecord");
+  }
+  if (launch_begin(plan, ctx->cam_host, n_inline, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "svo_composite: frame begin launch");
   if (launch_composite(p, plan.shape, adjoint ? kModeAdjoint : kModeComposite, ctx->stream) != 0)
     return hip_fail(hipGetLastError(), "svo_composite: launch");
   ctx->last_rays = N;

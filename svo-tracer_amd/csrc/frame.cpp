@@ -314,24 +314,41 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   return SVO_OK;
 }
 
-int launch_schedule(const BatchPlan &plan, hipStream_t stream) {
-  if (!plan.p.order) return 0;
-  return launch_tile_order(plan.p.tile_cost, (uint32_t *)plan.p.order, plan.p.range_begin, stream);
+int launch_begin(const BatchPlan &plan, const CamRec *host_cams, uint32_t n_inline, hipStream_t stream) {
+  FrameBegin f;
+  std::memset(&f, 0, sizeof f);
+  f.counters = plan.p.counters;
+  f.totals = plan.p.totals;
+  f.cams = const_cast<CamRec *>(plan.p.cams);
+  f.n_cams = n_inline;
+  for (uint32_t c = 0; c < n_inline; c++) f.cam[c] = host_cams[c];
+  if (plan.p.order) {
+    f.bands = 8;
+    f.cost = plan.p.tile_cost;
+    f.order = const_cast<uint32_t *>(plan.p.order);
+    std::memcpy(f.range_begin, plan.p.range_begin, sizeof f.range_begin);
+  }
+  return launch_frame_begin(f, stream);
 }
 
-// The stream work of one planned frame: camera records, counter reset, trace
-// kernel, compaction.  No host synchronisation and no memset nodes (capturable).
-int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool timing) {
-  if (plan.n_cams) {
+// The stream work of one planned frame: camera records, counter reset and tile order (one
+// launch), trace kernel, compaction.  No host synchronisation and no memset nodes
+// (capturable).  eager: submitted now, not captured; then up to kInlineCams camera records
+// ride in the first launch's argument (the staging records are not read after the call
+// returns), more are copied and ctx->cam_copied follows the copy, and the trace kernel is
+// timed when kernel timing is on.
+int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool eager) {
+  const uint32_t n_inline = eager && plan.n_cams <= kInlineCams ? plan.n_cams : 0u;
+  if (plan.n_cams && !n_inline) {
     HIP_TRY(hipMemcpyAsync((void *)plan.p.cams, host_cams, plan.n_cams * sizeof(CamRec), hipMemcpyHostToDevice,
                            ctx->stream),
             "camera upload");
+    if (eager) HIP_TRY(hipEventRecord(ctx->cam_copied, ctx->stream), "hipEventRecord");
   }
-  if (launch_frame_prepare(plan.p.counters, plan.p.totals, ctx->stream) != 0)
-    return hip_fail(hipGetLastError(), "trace frame: prepare launch");
-  if (launch_schedule(plan, ctx->stream) != 0) return hip_fail(hipGetLastError(), "trace frame: tile order launch");
+  if (launch_begin(plan, host_cams, n_inline, ctx->stream) != 0)
+    return hip_fail(hipGetLastError(), "trace frame: frame begin launch");
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
-  if (timing && ctx->kernel_timing) {
+  if (eager && ctx->kernel_timing) {
     if (ctx->kernel_events_used == ctx->kernel_events.size()) {
       std::pair<hipEvent_t, hipEvent_t> pr;
       HIP_TRY(hipEventCreate(&pr.first), "hipEventCreate");
@@ -367,7 +384,6 @@ int trace_batch_impl(Context *ctx, const svo_trace_input *ins, const float *aper
                        (CamRec *)ctx->frame_scratch.cams->ptr, nullptr, nullptr, &plan)))
     return rc;
   if ((rc = submit_batch(ctx, plan, ctx->cam_host, true))) return rc;
-  HIP_TRY(hipEventRecord(ctx->cam_copied, ctx->stream), "hipEventRecord");
   ctx->last_rays = plan.cones;
   return SVO_OK;
 }

@@ -91,14 +91,28 @@ struct CompactParams {
 };
 
 int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu);
-// Orders each band's tiles by the cost its waves measured in the previous frame (most
-// expensive first, in power-of-two classes, stable within a class).
-int launch_tile_order(const uint32_t *tile_cost, uint32_t *order, const uint32_t range_begin[9], hipStream_t stream);
+// What a frame does before its trace launch, in one launch: clear the counters and totals,
+// write up to kInlineCams camera records carried in the argument (more are uploaded by a
+// copy beforehand), and, with bands = 8, order each band's tiles by the cost its waves
+// measured in the previous frame (most expensive first, in power-of-two classes, stable
+// within a class).
+constexpr uint32_t kInlineCams = 4;
+struct FrameBegin {
+  uint32_t *counters;
+  unsigned long long *totals;
+  CamRec *cams;     // destination of the inline records
+  uint32_t n_cams;  // inline records to write, 0..kInlineCams
+  uint32_t bands;   // 0 (no tile order) or 8
+  const uint32_t *cost;
+  uint32_t *order;
+  uint32_t range_begin[9];
+  CamRec cam[kInlineCams];
+};
+int launch_frame_begin(const FrameBegin &f, hipStream_t stream);
 // Sets n words to zero (a kernel, so it may be captured in a graph).
 int launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t stream);
 // kind 0: floatToUlong of n floats into n u64; kind 1: ulongToFloat of n u64 into n floats.
 int launch_convert(uint32_t kind, const void *in, void *out, size_t n, hipStream_t stream);
-int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream);
 int launch_trace_frame(const FrameParams &p, const LaunchShape &shape, hipStream_t stream);
 // Compositing modes of the trace kernel (camera table, 256-lane work-groups).
 enum { kModePlain = 0, kModeComposite = 1, kModeAdjoint = 2 };

@@ -79,12 +79,14 @@ struct FrameScratch {
   std::shared_ptr<DeviceBuffer> tile_cost, tile_order;
   uint32_t sched_ranges[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
-// Launches the tile-order kernel of a planned frame (no-op without a schedule).
-int launch_schedule(const BatchPlan &plan, hipStream_t stream);
+// One launch before a planned frame's trace: clears its counters and totals, writes the first
+// n_inline of host_cams (n_inline <= kInlineCams) to the device records, and orders the
+// tiles when the plan has a tile order.
+int launch_begin(const BatchPlan &plan, const CamRec *host_cams, uint32_t n_inline, hipStream_t stream);
 int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
                const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
                const float *lut_dev, const uint32_t *lut_offsets, BatchPlan *plan);
-int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool timing);
+int submit_batch(Context *ctx, const BatchPlan &plan, const CamRec *host_cams, bool eager);
 int validate_batch(const svo_trace_input *ins, const float *apertures, uint32_t n_cams, size_t *cones);
 void fill_camera_poses(const svo_trace_input *ins, const float *apertures, uint32_t n_cams, CamRec *cams);
 // Several cameras in one persistent launch; outputs concatenated camera by camera.

@@ -508,10 +508,6 @@ trace_frame_kernel(const FrameParams p) {
 
 // ---- tile schedule ----
 
-struct Ranges {
-  uint32_t v[9];
-};
-
 // Cost class of a tile: floor(log2(iterations)), clamped to 15; 0 for an unmeasured tile.
 __device__ __forceinline__ uint32_t cost_class(uint32_t c) { return c ? min(15u, 31u - (uint32_t)__clz(c)) : 0u; }
 
@@ -521,10 +517,9 @@ constexpr uint32_t kOrderThreads = 1024;
 // expensive class first, so that the tiles that ran longest in the previous frame start
 // first and the frame does not end waiting on an expensive tile taken late.  Thread t
 // owns a contiguous run of the band's tiles.
-__global__ void __launch_bounds__(kOrderThreads) tile_order_kernel(const uint32_t *cost, uint32_t *order, Ranges rb) {
+__device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order, uint32_t b0, uint32_t n) {
   __shared__ uint32_t s_wave[16][kOrderThreads / 64];
   __shared__ uint32_t s_base[16];
-  const uint32_t b0 = rb.v[blockIdx.x], n = rb.v[blockIdx.x + 1] - b0;
   const uint32_t per = (n + kOrderThreads - 1) / kOrderThreads;
   const uint32_t i0 = min(n, threadIdx.x * per), i1 = min(n, i0 + per);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -585,6 +580,21 @@ __global__ void __launch_bounds__(kOrderThreads) tile_order_kernel(const uint32_
   }
 }
 
+// The work a frame does before its trace launch, in one launch: work-group 0 clears the
+// frame's counters and totals and writes the camera records that came in the kernel
+// argument; work-groups 0..bands-1 order their band's tiles (above).
+__global__ void __launch_bounds__(kOrderThreads) frame_begin_kernel(FrameBegin f) {
+  if (blockIdx.x == 0) {
+    const uint32_t i = threadIdx.x;
+    if (i < 8u) f.counters[i * kCounterStride] = 0u;
+    if (i < 4u) f.totals[i] = 0ull;
+    if (i < f.n_cams * kCamWords)
+      reinterpret_cast<uint32_t *>(f.cams)[i] = reinterpret_cast<const uint32_t *>(f.cam)[i];
+  }
+  if (blockIdx.x < f.bands) order_band(f.cost, f.order, f.range_begin[blockIdx.x],
+                                       f.range_begin[blockIdx.x + 1] - f.range_begin[blockIdx.x]);
+}
+
 __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
@@ -598,12 +608,6 @@ __global__ void convert_kernel(uint32_t kind, const void *in, void *out, size_t 
     else
       static_cast<float *>(out)[i] = ulong_to_float(static_cast<const uint64_t *>(in)[i]);
   }
-}
-
-__global__ void frame_prepare_kernel(uint32_t *counters, unsigned long long *totals) {
-  const uint32_t i = threadIdx.x;
-  if (i < 8u) counters[i * kCounterStride] = 0u;
-  if (i < 4u) totals[i] = 0ull;
 }
 
 // ---- order-preserving status compaction (ascending cone id) ----
@@ -788,10 +792,9 @@ int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
   }
 }
 
-int launch_tile_order(const uint32_t *tile_cost, uint32_t *order, const uint32_t range_begin[9], hipStream_t stream) {
-  Ranges r;
-  for (int i = 0; i < 9; i++) r.v[i] = range_begin[i];
-  hipLaunchKernelGGL(tile_order_kernel, dim3(8), dim3(kOrderThreads), 0, stream, tile_cost, order, r);
+int launch_frame_begin(const FrameBegin &f, hipStream_t stream) {
+  if (f.n_cams > kInlineCams || f.bands > 8u) return -1;
+  hipLaunchKernelGGL(frame_begin_kernel, dim3(std::max(1u, f.bands)), dim3(kOrderThreads), 0, stream, f);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -806,11 +809,6 @@ int launch_convert(uint32_t kind, const void *in, void *out, size_t n, hipStream
   if (n == 0) return 0;
   const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(convert_kernel, dim3(blocks), dim3(256), 0, stream, kind, in, out, n);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_frame_prepare(uint32_t *counters, unsigned long long *totals, hipStream_t stream) {
-  hipLaunchKernelGGL(frame_prepare_kernel, dim3(1), dim3(64), 0, stream, counters, totals);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
