@@ -31,7 +31,7 @@ struct svo_terrain {
 
 namespace {
 
-enum StreamBuf { kSbKeys, kSbVals, kSbFlags, kSbRank, kSbCounts, kSbFrom, kSbTo, kSbRemap, kSbCount };
+enum StreamBuf { kSbKeys, kSbVals, kSbFlags, kSbRank, kSbList, kSbCounts, kSbFrom, kSbTo, kSbRemap, kSbCount };
 enum : uint32_t { kShTotal = 0, kShSlots = 16 };
 
 constexpr uint64_t kUsageInviolate = 0xFFFFFFFF0000FFFFull;  // tick 65535, parent UINT32_MAX
@@ -315,7 +315,7 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     out.requests = count;
   }
   // 2. distinct requesting blocks that can take a child level, ranked in request order
-  uint32_t unique = 0, *flags = nullptr, *rank = nullptr;
+  uint32_t unique = 0, *flags = nullptr, *rank = nullptr, *list = nullptr;
   if (n) {
     uint32_t cap = 64;
     while (cap < 2ull * n && cap < (1u << 31)) cap <<= 1;
@@ -328,6 +328,8 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     flags = (uint32_t *)p;
     if ((rc = st.grow(kSbRank, (size_t)n * 4, &p))) return rc;
     rank = (uint32_t *)p;
+    if ((rc = st.grow(kSbList, (size_t)std::max(1u, std::min(n, max_bricks)) * 4, &p))) return rc;
+    list = (uint32_t *)p;
     if (tree_fill_u32(keys, 0xffffffffu, cap, s) || tree_fill_u32(vals, 0xffffffffu, cap, s))
       return fail(SVO_ERR_HIP, "stream: fill");
     if ((rc = st.launch(launch_request_unique((const svo_child_request *)b.requests->ptr, n, keys, vals, cap - 1u,
@@ -337,7 +339,9 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     const uint32_t chunks = (n + 1023u) / 1024u;
     if ((rc = st.grow(kSbCounts, ((size_t)chunks + kShSlots) * 4, &p))) return rc;
     st.counters = (uint32_t *)p + chunks;
-    if ((rc = st.launch(launch_rank(flags, n, (uint32_t *)p, rank, st.counters + kShTotal, s), "rank"))) return rc;
+    if ((rc = st.launch(launch_rank(flags, n, (uint32_t *)p, rank, st.counters + kShTotal, s, list, max_bricks),
+                        "rank")))
+      return rc;
     if ((rc = st.read_u32(st.counters + kShTotal, &unique))) return rc;
   }
   out.unique = unique;
@@ -353,7 +357,7 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     std::shared_ptr<DeviceBuffer> rec, gb;
     if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING, (size_t)nb * sizeof(svo_grafting), &rec))) return rc;
     if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING_BLOCKS, (size_t)nb * 8 * sizeof(svo_block), &gb))) return rc;
-    if ((rc = st.launch(launch_bricks(t->view, (const svo_child_request *)b.requests->ptr, n, flags, rank, nb,
+    if ((rc = st.launch(launch_bricks(t->view, (const svo_child_request *)b.requests->ptr, list, nb,
                                       (svo_grafting *)rec->ptr, (uint4 *)gb->ptr, s),
                         "bricks")))
       return rc;

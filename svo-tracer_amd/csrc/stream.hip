@@ -237,16 +237,13 @@ __global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_reque
   flags[i] = ok ? 1u : 0u;
 }
 
-// Request i (flagged, rank k) becomes graft record k and brick blocks [8k, 8k+8).
-__global__ __launch_bounds__(kWg) void brick_kernel(TerrainView t, const svo_child_request *req, uint32_t n,
-                                                    const uint32_t *flags, const uint32_t *rank, uint32_t limit,
-                                                    svo_grafting *records, uint4 *bricks) {
+// Request list[k] becomes graft record k and brick blocks [8k, 8k+8); eight threads a brick.
+__global__ __launch_bounds__(kWg) void brick_kernel(TerrainView t, const svo_child_request *req, const uint32_t *list,
+                                                    uint32_t nb, svo_grafting *records, uint4 *bricks) {
   const uint32_t tid = blockIdx.x * kWg + threadIdx.x;
-  const uint32_t i = tid >> 3, c = tid & 7u;
-  if (i >= n || !flags[i]) return;
-  const uint32_t k = rank[i];
-  if (k >= limit) return;
-  const svo_child_request r = req[i];
+  const uint32_t k = tid >> 3, c = tid & 7u;
+  if (k >= nb) return;
+  const svo_child_request r = req[list[k]];
   const uint32_t D = r.depth, s = 64u - D;
   const uint32_t x = (uint32_t)(r.location[0] >> s), y = (uint32_t)(r.location[1] >> s),
                  z = (uint32_t)(r.location[2] >> s);
@@ -336,8 +333,9 @@ __global__ __launch_bounds__(kWg) void flag_count_kernel(const uint32_t *flags, 
   if (threadIdx.x == 0) counts[blockIdx.x] = all;
 }
 
+// rank[i] = flagged entries before i; with a list, flagged i of rank r < limit is list[r].
 __global__ __launch_bounds__(kWg) void flag_rank_kernel(const uint32_t *flags, uint32_t n, const uint32_t *offsets,
-                                                        uint32_t *rank) {
+                                                        uint32_t *rank, uint32_t *list, uint32_t limit) {
   __shared__ uint32_t s_waves[kWg / 64];
   const uint32_t i0 = blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
   uint32_t f[kSelPer], c = 0;
@@ -349,6 +347,7 @@ __global__ __launch_bounds__(kWg) void flag_rank_kernel(const uint32_t *flags, u
   uint32_t r = offsets[blockIdx.x] + wg_scan<kWg>(c, s_waves, &all);
   for (uint32_t j = 0; j < kSelPer; j++) {
     if (i0 + j < n) rank[i0 + j] = r;
+    if (list && f[j] && r < limit) list[r] = i0 + j;
     r += f[j];
   }
 }
@@ -684,21 +683,22 @@ int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *ke
   return rc_of();
 }
 
-int launch_bricks(const TerrainView &t, const svo_child_request *req, uint32_t n, const uint32_t *flags,
-                  const uint32_t *rank, uint32_t limit, svo_grafting *records, uint4 *bricks, hipStream_t s) {
-  if (!n || !limit) return 0;
-  hipLaunchKernelGGL(brick_kernel, dim3((uint32_t)((8ull * n + kWg - 1) / kWg)), dim3(kWg), 0, s, t, req, n, flags,
-                     rank, limit, records, bricks);
+int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb,
+                  svo_grafting *records, uint4 *bricks, hipStream_t s) {
+  if (!nb) return 0;
+  hipLaunchKernelGGL(brick_kernel, dim3((uint32_t)((8ull * nb + kWg - 1) / kWg)), dim3(kWg), 0, s, t, req, list, nb,
+                     records, bricks);
   return rc_of();
 }
 
-int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s) {
+int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s,
+                uint32_t *list, uint32_t limit) {
   const uint32_t chunks = (n + kSelChunk - 1) / kSelChunk;
   if (!chunks) return 0;
   hipLaunchKernelGGL(flag_count_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts);
   if (rc_of()) return -1;
   if (launch_scan_u32(counts, counts, chunks, total, s)) return -1;
-  hipLaunchKernelGGL(flag_rank_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts, rank);
+  hipLaunchKernelGGL(flag_rank_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts, rank, list, limit);
   return rc_of();
 }
 

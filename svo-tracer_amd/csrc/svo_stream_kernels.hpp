@@ -35,12 +35,13 @@ int launch_seed(const TerrainView &t, uint16_t *bases, uint32_t B, uint4 *blocks
 int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t mask,
                           const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s);
-// Flagged request i with rank[i] < limit becomes graft record rank[i] and brick blocks
-// [8 rank[i], 8 rank[i] + 8).
-int launch_bricks(const TerrainView &t, const svo_child_request *req, uint32_t n, const uint32_t *flags,
-                  const uint32_t *rank, uint32_t limit, svo_grafting *records, uint4 *bricks, hipStream_t s);
-// rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.
-int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s);
+// Request list[k] (k < nb) becomes graft record k and brick blocks [8k, 8k + 8).
+int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb,
+                  svo_grafting *records, uint4 *bricks, hipStream_t s);
+// rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.  With
+// a list, the flagged entries of rank below limit are listed in rank order.
+int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s,
+                uint32_t *list = nullptr, uint32_t limit = 0);
 // Groups of [lo, hi) that satisfy a predicate, in ascending order: per-chunk counts, then the
 // scatter of the list given the scanned counts.
 uint32_t select_chunks(uint32_t lo, uint32_t hi);

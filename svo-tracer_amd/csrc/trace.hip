@@ -704,39 +704,55 @@ __global__ void __launch_bounds__(kCompactThreads) status_scan_kernel(uint32_t *
   }
 }
 
+// Number of set bits of m below this lane.
+__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Wave w of a work-group writes the entries of cone ids [chunk + 1024 w, chunk + 1024 (w+1))
+// in 16 rounds of 64 consecutive ids, one per lane: each round's entries of a queue go to
+// consecutive positions (ranks from ballots), so the record reads and the queue writes are
+// coalesced.  Positions continue the chunk offsets of the scan in ascending cone id.
 __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
     const uint8_t *status, uint32_t n, const uint32_t *offsets, uint32_t *bg_queue, uint32_t *mat_queue,
     const svo_child_request *req_sparse, svo_child_request *requests, uint32_t req_capacity) {
-  const uint32_t base = blockIdx.x * kCompactChunk + threadIdx.x * kCompactPerThread;
-  uint8_t s[16];
-  load_status16(status, n, base, s);
-  uint32_t v[3] = {0u, 0u, 0u};
+  constexpr uint32_t kRounds = kCompactChunk / kCompactThreads;
+  __shared__ uint32_t s_tot[3][kCompactThreads / 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t wbase = blockIdx.x * kCompactChunk + wave * (64u * kRounds);
+  uint64_t cats = 0;  // 3 category bits per round
+  uint32_t tot[3] = {0u, 0u, 0u};
 #pragma unroll
-  for (int i = 0; i < 16; i++) {
-    const uint32_t c = categories(s[i]);
-    v[0] += c & 1u;
-    v[1] += (c >> 1) & 1u;
-    v[2] += (c >> 2) & 1u;
+  for (uint32_t r = 0; r < kRounds; r++) {
+    const uint32_t id = wbase + r * 64u + lane;
+    const uint32_t c = categories(id < n ? status[id] : (uint8_t)0xffu);
+    cats |= (uint64_t)c << (3u * r);
+#pragma unroll
+    for (uint32_t q = 0; q < 3; q++) tot[q] += (uint32_t)__popcll(__ballot((c >> q) & 1u));
   }
-  uint32_t total[3];
-  block_scan3(v, total);
+  if (lane == 0)
+    for (uint32_t q = 0; q < 3; q++) s_tot[q][wave] = tot[q];
+  __syncthreads();
   uint32_t pos[3];
-  for (int c = 0; c < 3; c++) pos[c] = offsets[4u * blockIdx.x + c] + v[c];
-  for (int i = 0; i < 16; i++) {
-    const uint32_t c = categories(s[i]);
-    const uint32_t id = base + i;
-    if (c & 1u) {
-      if (bg_queue) bg_queue[pos[0]] = id;
-      pos[0]++;
-    }
-    if (c & 2u) {
-      if (mat_queue) mat_queue[pos[1]] = id;
-      pos[1]++;
-    }
+#pragma unroll
+  for (uint32_t q = 0; q < 3; q++) {
+    pos[q] = offsets[4u * blockIdx.x + q];
+    for (uint32_t w = 0; w < wave; w++) pos[q] += s_tot[q][w];
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < kRounds; r++) {
+    const uint32_t id = wbase + r * 64u + lane;
+    const uint32_t c = (uint32_t)(cats >> (3u * r)) & 7u;
+    const uint64_t m0 = __ballot(c & 1u), m1 = __ballot(c & 2u), m2 = __ballot(c & 4u);
+    if ((c & 1u) && bg_queue) bg_queue[pos[0] + rank_below(m0)] = id;
+    if ((c & 2u) && mat_queue) mat_queue[pos[1] + rank_below(m1)] = id;
     if (c & 4u) {
-      if (requests && pos[2] < req_capacity) requests[pos[2]] = req_sparse[id];
-      pos[2]++;
+      const uint32_t at = pos[2] + rank_below(m2);
+      if (requests && at < req_capacity) requests[at] = req_sparse[id];
     }
+    pos[0] += (uint32_t)__popcll(m0);
+    pos[1] += (uint32_t)__popcll(m1);
+    pos[2] += (uint32_t)__popcll(m2);
   }
 }
 

@@ -7,6 +7,8 @@
 // target, the record with the highest index wins (a hash table keyed by (field, target)
 // holding the largest record index).  Collection ranks candidates in ascending group
 // order with an order-preserving count / scan / scatter, one pass per age threshold.
+#include <mutex>
+
 #include "svo_device.hpp"
 #include "svo_tree_kernels.hpp"
 
@@ -162,19 +164,88 @@ __global__ void fill_u64_kernel(unsigned long long *p, unsigned long long v, siz
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Exclusive scans over spans of kScanSpan elements.  Wave w of the work-group owns 64 *
+// kScanPer consecutive elements of a span and reads and writes them in kScanPer rounds of
+// 64 consecutive elements (coalesced); the rounds' sums chain within the wave and the wave
+// sums chain over the work-group.  in may equal out: each element is read and written by
+// the same lane.
+constexpr uint32_t kScanPer = 16;
+constexpr uint32_t kScanSpan = kScanWg * kScanPer;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(x, d, 64);
+    if ((int)lane >= d) x += t;
+  }
+  return x;
+}
+
+// Scans [s0, s0 + kScanSpan) clipped to n, starting from base; returns the span's sum.
+__device__ uint32_t scan_span(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t s0, uint32_t base,
+                              uint32_t *s_waves) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t w0 = s0 + wave * (64u * kScanPer);
+  uint32_t v[kScanPer], incl[kScanPer], before[kScanPer];
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; r++) {
+    const uint32_t i = w0 + r * 64u + lane;
+    v[r] = i < n ? in[i] : 0u;
+  }
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; r++) {
+    incl[r] = wave_incl_scan(v[r]);
+    before[r] = run;
+    run += (uint32_t)__builtin_amdgcn_readlane((int)incl[r], 63);
+  }
+  if (lane == 0) s_waves[wave] = run;
+  __syncthreads();
+  uint32_t wbefore = 0, all = 0;
+  for (uint32_t w = 0; w < (uint32_t)(kScanWg / 64); w++) {
+    const uint32_t t = s_waves[w];
+    wbefore += w < wave ? t : 0u;
+    all += t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; r++) {
+    const uint32_t i = w0 + r * 64u + lane;
+    if (i < n) out[i] = base + wbefore + before[r] + incl[r] - v[r];
+  }
+  return all;
+}
+
+// Single-work-group exclusive scan of n elements.
 __global__ __launch_bounds__(kScanWg) void scan_u32_kernel(const uint32_t *in, uint32_t *out, uint32_t n,
                                                            uint32_t *total) {
   __shared__ uint32_t s_waves[kScanWg / 64];
   uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < n; c0 += kScanWg) {
-    const uint32_t i = c0 + threadIdx.x;
-    const uint32_t v = i < n ? in[i] : 0u;
-    uint32_t all;
-    const uint32_t pre = block_scan<kScanWg>(v, s_waves, &all);
-    if (i < n) out[i] = carry + pre;
-    carry += all;
-  }
+  for (uint32_t s0 = 0; s0 < n; s0 += kScanSpan) carry += scan_span(in, out, n, s0, carry, s_waves);
   if (threadIdx.x == 0) *total = carry;
+}
+
+// Multi-work-group scan of a large array: block b sums span b, one work-group scans the
+// span sums, block b scans span b from its offset.
+__global__ __launch_bounds__(kScanWg) void scan_reduce_kernel(const uint32_t *in, uint32_t n, uint32_t *partial) {
+  __shared__ uint32_t s_waves[kScanWg / 64];
+  const uint32_t s0 = blockIdx.x * kScanSpan;
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kScanPer; r++) {
+    const uint32_t i = s0 + r * kScanWg + threadIdx.x;
+    sum += i < n ? in[i] : 0u;
+  }
+  uint32_t all;
+  (void)block_scan<kScanWg>(sum, s_waves, &all);
+  if (threadIdx.x == 0) partial[blockIdx.x] = all;
+}
+
+__global__ __launch_bounds__(kScanWg) void scan_apply_kernel(const uint32_t *in, uint32_t *out, uint32_t n,
+                                                             const uint32_t *offset) {
+  __shared__ uint32_t s_waves[kScanWg / 64];
+  (void)scan_span(in, out, n, blockIdx.x * kScanSpan, offset[blockIdx.x], s_waves);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,15 +303,35 @@ __global__ __launch_bounds__(kScanWg) void request_scan_kernel(PoolView p, Recor
   if (threadIdx.x == 0) s_base = p.request_count ? *p.request_count : 0u;
   __syncthreads();
   const uint32_t base = s_base;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint32_t carry = 0;
-  for (uint32_t c0 = 0; c0 < n; c0 += kScanWg) {
-    const uint32_t i = c0 + threadIdx.x;
-    const bool miss = i < n && res[i].status == kResMiss;
-    uint32_t all;
-    const uint32_t pre = block_scan<kScanWg>(miss ? 1u : 0u, s_waves, &all);
-    if (miss && p.request_count) {
-      const uint32_t id = base + carry + pre;
-      if (id < p.request_capacity) {
+  // wave w takes 64 * kScanPer consecutive records of each span in rounds of 64 (one a lane)
+  for (uint32_t s0 = 0; s0 < n; s0 += kScanSpan) {
+    const uint32_t w0 = s0 + wave * (64u * kScanPer);
+    uint64_t m[kScanPer];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; r++) {
+      const uint32_t i = w0 + r * 64u + lane;
+      m[r] = __ballot(i < n && res[i].status == kResMiss);
+      cnt += (uint32_t)__popcll(m[r]);
+    }
+    if (lane == 0) s_waves[wave] = cnt;
+    __syncthreads();
+    uint32_t wbefore = 0, all = 0;
+    for (uint32_t w = 0; w < (uint32_t)(kScanWg / 64); w++) {
+      const uint32_t t = s_waves[w];
+      wbefore += w < wave ? t : 0u;
+      all += t;
+    }
+    __syncthreads();
+    uint32_t id = base + carry + wbefore;
+#pragma unroll
+    for (uint32_t r = 0; r < kScanPer; r++) {
+      const uint32_t i = w0 + r * 64u + lane;
+      const uint32_t at = id + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[r] >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m[r], 0u));
+      if (((m[r] >> lane) & 1ull) && p.request_count && at < p.request_capacity) {
         const ResolveOut o = res[i];
         const uint64_t *loc = locs + 3ull * f.address(i);
         svo_child_request q;
@@ -253,8 +344,9 @@ __global__ __launch_bounds__(kScanWg) void request_scan_kernel(PoolView p, Recor
         q.location[2] = loc[2];
         q.tree_size = (uint8_t)(f.depth(i) - o.level);
         for (int k = 0; k < 7; k++) q.pad1[k] = 0;
-        p.requests[id] = q;
+        p.requests[at] = q;
       }
+      id += (uint32_t)__popcll(m[r]);
     }
     carry += all;
   }
@@ -695,8 +787,34 @@ int tree_fill_u64(unsigned long long *p, unsigned long long value, size_t n, hip
   return launch_rc();
 }
 
+// Span sums of the multi-work-group scan: one array per device, allocated on the first
+// large scan and kept until the process ends.  Scans on one device are stream-ordered
+// (every caller submits on its context's stream), so one array serves them all.
+constexpr uint32_t kMaxSpans = (uint32_t)((1ull << 32) / kScanSpan);
+static uint32_t *scan_partials(int device) {
+  static std::mutex mu;
+  static uint32_t *bufs[64] = {};
+  if (device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!bufs[device] && hipMalloc((void **)&bufs[device], (size_t)kMaxSpans * 4) != hipSuccess) bufs[device] = nullptr;
+  return bufs[device];
+}
+
 int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, hipStream_t s) {
-  hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(kScanWg), 0, s, in, out, n, total);
+  if (n <= 2u * kScanSpan) {
+    hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(kScanWg), 0, s, in, out, n, total);
+    return launch_rc();
+  }
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return -1;
+  uint32_t *partial = scan_partials(device);
+  if (!partial) return -1;
+  const uint32_t spans = (uint32_t)(((uint64_t)n + kScanSpan - 1) / kScanSpan);
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(spans), dim3(kScanWg), 0, s, in, n, partial);
+  if (launch_rc()) return -1;
+  hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(kScanWg), 0, s, partial, partial, spans, total);
+  if (launch_rc()) return -1;
+  hipLaunchKernelGGL(scan_apply_kernel, dim3(spans), dim3(kScanWg), 0, s, in, out, n, partial);
   return launch_rc();
 }
 
