@@ -103,6 +103,7 @@ struct FrameBegin {
   CamRec *cams;     // destination of the inline records
   uint32_t n_cams;  // inline records to write, 0..kInlineCams
   uint32_t bands;   // 0 (no tile order) or 8
+  uint32_t stage;   // set by the launcher: stage the cost classes in LDS
   const uint32_t *cost;
   uint32_t *order;
   uint32_t range_begin[9];

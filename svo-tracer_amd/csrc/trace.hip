@@ -517,9 +517,29 @@ constexpr uint32_t kOrderThreads = 1024;
 // expensive class first, so that the tiles that ran longest in the previous frame start
 // first and the frame does not end waiting on an expensive tile taken late.  Thread t
 // owns a contiguous run of the band's tiles.
-__device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order, uint32_t b0, uint32_t n) {
+// With cls (LDS of at least n bytes) the band's cost classes are staged there first, eight
+// loads a thread in flight together, and both passes read them from LDS; without it each
+// pass reads the costs from memory, one load per tile in turn.
+__device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order, uint32_t b0, uint32_t n,
+                                           uint8_t *cls) {
   __shared__ uint32_t s_wave[16][kOrderThreads / 64];
   __shared__ uint32_t s_base[16];
+  if (cls) {
+    for (uint32_t j0 = 0; j0 < n; j0 += 8u * kOrderThreads) {
+      uint32_t c[8];
+#pragma unroll
+      for (uint32_t t = 0; t < 8u; t++) {
+        const uint32_t i = j0 + t * kOrderThreads + threadIdx.x;
+        c[t] = i < n ? cost[b0 + i] : 0u;
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < 8u; t++) {
+        const uint32_t i = j0 + t * kOrderThreads + threadIdx.x;
+        if (i < n) cls[i] = (uint8_t)cost_class(c[t]);
+      }
+    }
+    __syncthreads();
+  }
   const uint32_t per = (n + kOrderThreads - 1) / kOrderThreads;
   const uint32_t i0 = min(n, threadIdx.x * per), i1 = min(n, i0 + per);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -527,7 +547,7 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
 #pragma unroll
   for (int c = 0; c < 16; c++) cnt[c] = 0;
   for (uint32_t i = i0; i < i1; i++) {
-    const uint32_t k = cost_class(cost[b0 + i]);
+    const uint32_t k = cls ? (uint32_t)cls[i] : cost_class(cost[b0 + i]);
 #pragma unroll
     for (int c = 0; c < 16; c++) cnt[c] += (k == (uint32_t)c) ? 1u : 0u;
   }
@@ -567,7 +587,7 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
 #pragma unroll
   for (int c = 0; c < 16; c++) off[c] += s_base[c] + s_wave[c][wave];
   for (uint32_t i = i0; i < i1; i++) {
-    const uint32_t k = cost_class(cost[b0 + i]);
+    const uint32_t k = cls ? (uint32_t)cls[i] : cost_class(cost[b0 + i]);
     uint32_t pos = 0;
 #pragma unroll
     for (int c = 0; c < 16; c++) {
@@ -591,8 +611,10 @@ __global__ void __launch_bounds__(kOrderThreads) frame_begin_kernel(FrameBegin f
     if (i < f.n_cams * kCamWords)
       reinterpret_cast<uint32_t *>(f.cams)[i] = reinterpret_cast<const uint32_t *>(f.cam)[i];
   }
+  extern __shared__ uint8_t s_cls[];
   if (blockIdx.x < f.bands) order_band(f.cost, f.order, f.range_begin[blockIdx.x],
-                                       f.range_begin[blockIdx.x + 1] - f.range_begin[blockIdx.x]);
+                                       f.range_begin[blockIdx.x + 1] - f.range_begin[blockIdx.x],
+                                       f.stage ? s_cls : nullptr);
 }
 
 __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
@@ -810,7 +832,14 @@ int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
 
 int launch_frame_begin(const FrameBegin &f, hipStream_t stream) {
   if (f.n_cams > kInlineCams || f.bands > 8u) return -1;
-  hipLaunchKernelGGL(frame_begin_kernel, dim3(std::max(1u, f.bands)), dim3(kOrderThreads), 0, stream, f);
+  // bands of up to kStageTiles tiles stage their cost classes in LDS (one byte a tile)
+  constexpr uint32_t kStageTiles = 48u * 1024u;
+  uint32_t most = 0;
+  for (uint32_t b = 0; b < f.bands; b++) most = std::max(most, f.range_begin[b + 1] - f.range_begin[b]);
+  FrameBegin g = f;
+  g.stage = most > 0u && most <= kStageTiles ? 1u : 0u;
+  const size_t lds = g.stage ? (size_t)((most + 15u) & ~15u) : 0;
+  hipLaunchKernelGGL(frame_begin_kernel, dim3(std::max(1u, f.bands)), dim3(kOrderThreads), lds, stream, g);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
