@@ -114,8 +114,9 @@ int launch_graft_structure(const svo_grafting *rec, uint32_t n, const svo_block 
 int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, hipStream_t s);
 int launch_graft_winners(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
 int launch_graft_roots(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
-int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, const svo_block *gb,
-                       const TreeScratch &t, hipStream_t s);
+// total_need: the sum of need over the n records (the scan total of need into start).
+int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, uint32_t total_need,
+                       const svo_block *gb, const TreeScratch &t, hipStream_t s);
 int launch_graft_release(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
 
 }  // namespace svo

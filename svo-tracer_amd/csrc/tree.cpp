@@ -308,7 +308,7 @@ int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
   if ((rc = op.launch(launch_graft_winners(pv, n, got, ts, s), "graft winners"))) return rc;
   if ((rc = op.launch(launch_graft_roots(pv, n, got, ts, s), "graft roots"))) return rc;
   if ((rc = op.free_walk(true))) return rc;
-  if ((rc = op.launch(launch_graft_place(pv, rec, n, gb, ts, s), "graft place"))) return rc;
+  if ((rc = op.launch(launch_graft_place(pv, rec, n, total_need, gb, ts, s), "graft place"))) return rc;
   if ((rc = op.launch(launch_graft_release(pv, n, got, ts, s), "graft release"))) return rc;
   return op.stats(stats, got);
 }

@@ -715,18 +715,28 @@ __global__ __launch_bounds__(kFlat) void graft_roots_kernel(PoolView p, ResolveO
   if (c != kU32Max) push(front, &counters[kTcFront0], cap, &counters[kTcOverflow], c >> 3, o.addr);
 }
 
-// One work-group per graft: copies its blocks into the collected groups (Child fields
-// remapped) and sets the usage of every group.
+// One thread per needed block of all grafts: copies the blocks into the collected groups
+// (Child fields remapped) and sets the usage of every group.  Thread t places block t & 7
+// of needed group g = t / 8, which belongs to the last graft r with start[r] <= g (start is
+// the exclusive scan of need, so g < start[r] + need[r] for that r).
 __global__ __launch_bounds__(kFlat) void graft_place_kernel(PoolView p, const svo_grafting *rec, const svo_block *gb,
                                                             const ResolveOut *res, const uint32_t *start,
-                                                            const uint32_t *need, const uint32_t *where,
-                                                            const uint32_t *number, const uint32_t *parent,
-                                                            const uint32_t *list) {
-  const uint32_t r = blockIdx.x;
+                                                            const uint32_t *where, const uint32_t *number,
+                                                            const uint32_t *parent, const uint32_t *list, uint32_t n,
+                                                            uint32_t total_need) {
+  const uint32_t t = blockIdx.x * kFlat + threadIdx.x;
+  const uint32_t g = t >> 3;
+  if (g >= total_need) return;
+  uint32_t lo = 0, hi = n;  // start[lo] <= g < start[hi] (start[n] taken as total_need)
+  while (hi - lo > 1u) {
+    const uint32_t mid = lo + (hi - lo) / 2u;
+    if (start[mid] <= g) lo = mid; else hi = mid;
+  }
+  const uint32_t r = lo;
   if (!res[r].placed) return;
   const uint32_t gda = rec[r].graft_data_address, S = start[r], W = where[r];
-  const uint32_t nb = 8u * need[r];
-  for (uint32_t b = threadIdx.x; b < nb; b += kFlat) {
+  {
+    const uint32_t b = 8u * (g - S) + (t & 7u);
     const uint32_t q = b >> 3, k = b & 7u;
     const uint32_t P = list[S + number[W + q]];
     uint4 v = reinterpret_cast<const uint4 *>(gb)[gda + b];
@@ -953,11 +963,14 @@ int launch_graft_roots(const PoolView &p, uint32_t n, uint32_t got, const TreeSc
   return launch_rc();
 }
 
-int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, const svo_block *gb,
-                       const TreeScratch &t, hipStream_t s) {
+int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, uint32_t total_need,
+                       const svo_block *gb, const TreeScratch &t, hipStream_t s) {
   if (!n) return 0;
-  hipLaunchKernelGGL(graft_place_kernel, dim3(n), dim3(kFlat), 0, s, p, rec, gb, t.res, t.start, t.need, t.where,
-                     t.number, t.parent, t.list);
+  if (total_need) {
+    const uint32_t blocks = (uint32_t)((8ull * total_need + kFlat - 1) / kFlat);
+    hipLaunchKernelGGL(graft_place_kernel, dim3(blocks), dim3(kFlat), 0, s, p, rec, gb, t.res, t.start, t.where,
+                       t.number, t.parent, t.list, n, total_need);
+  }
   if (launch_rc()) return -1;
   hipLaunchKernelGGL(graft_link_kernel, dim3(blocks_for(n)), dim3(kFlat), 0, s, p, t.res, t.start, t.list, n);
   return launch_rc();
