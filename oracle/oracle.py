@@ -65,6 +65,17 @@ class MaterialIO(C.Structure):
     ]
 
 
+class ShadeIO(C.Structure):
+    _fields_ = [
+        ("queue", C.c_void_p), ("n", C.c_uint32), ("n_rays", C.c_uint32),
+        ("directions", C.c_void_p), ("parents", C.c_void_p), ("weightings", C.c_void_p),
+        ("colour_r", C.c_void_p), ("colour_g", C.c_void_p), ("colour_b", C.c_void_p),
+        ("locations", C.c_void_p), ("depths", C.c_void_p),
+        ("final_r", C.c_void_p), ("final_g", C.c_void_p), ("final_b", C.c_void_p), ("final_w", C.c_void_p),
+        ("n_pixels", C.c_uint32),
+    ]
+
+
 class TreeStats(C.Structure):
     _fields_ = [(k, C.c_uint32) for k in ("applied", "skipped", "invalid", "requests", "collected", "placed", "unlinks")]
 
@@ -138,6 +149,12 @@ def lib() -> C.CDLL:
         L.orc_evaluate_material.restype = C.c_int
         L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_sincos.restype = None
+        L.orc_evaluate_background.argtypes = [C.c_uint8, C.POINTER(ShadeIO)]
+        L.orc_evaluate_background.restype = C.c_int
+        L.orc_resolve_remainders.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(ShadeIO)]
+        L.orc_resolve_remainders.restype = C.c_int
+        L.orc_draw_trace.argtypes = [C.c_uint32] + [C.c_void_p] * 5
+        L.orc_draw_trace.restype = None
         L.orc_tick_age.argtypes = [C.c_uint16, C.c_uint16]
         L.orc_tick_age.restype = C.c_uint32
         _lib = L
@@ -573,3 +590,49 @@ def evaluate_material(trace_input, queue: np.ndarray, root: dict, blocks: np.nda
     rc = lib().orc_evaluate_material(C.byref(trace_input), C.byref(io))
     if rc != 0:
         raise RuntimeError(f"orc_evaluate_material failed: {rc}")
+
+
+def _shade_io(queue: np.ndarray, traces: dict, final: dict) -> tuple:
+    q = np.ascontiguousarray(queue, np.uint32)
+    io = ShadeIO()
+    io.queue, io.n = q.ctypes.data, q.size
+    n_rays = min(traces["parents"].size, traces["weightings"].size)
+    for k in ("directions", "locations"):
+        if k in traces:
+            setattr(io, k, traces[k].ctypes.data)
+            n_rays = min(n_rays, traces[k].size // 4)
+    for k in ("depths", "colour_r", "colour_g", "colour_b"):
+        if traces.get(k) is not None:
+            setattr(io, k, traces[k].ctypes.data)
+            n_rays = min(n_rays, traces[k].size)
+    io.parents, io.weightings = traces["parents"].ctypes.data, traces["weightings"].ctypes.data
+    io.n_rays = n_rays
+    for k in ("final_r", "final_g", "final_b", "final_w"):
+        assert final[k].dtype == np.uint32 and final[k].flags["C_CONTIGUOUS"]
+        setattr(io, k, final[k].ctypes.data)
+    io.n_pixels = min(final[k].size for k in ("final_r", "final_g", "final_b", "final_w"))
+    return io, q
+
+
+def evaluate_background(ambient: int, queue: np.ndarray, traces: dict, final: dict) -> None:
+    """evaluateBackground over `queue` (wavefront.cl:1741-1797, with the light mix of cones of
+    weighting below 255): traces = directions (float4), parents, weightings, colour_r/g/b;
+    final = final_r/g/b/w (uint32 per pixel), accumulated in place."""
+    io, _q = _shade_io(queue, traces, final)
+    lib().orc_evaluate_background(C.c_uint8(ambient), C.byref(io))
+
+
+def resolve_remainders(base_depth: int, blocks: np.ndarray, queue: np.ndarray, traces: dict, final: dict) -> None:
+    """resolveRemainders over `queue` (wavefront.cl:1799-1833): traces = locations (u64x4),
+    depths, parents, weightings; final accumulated in place."""
+    io, _q = _shade_io(queue, traces, final)
+    lib().orc_resolve_remainders(base_depth, blocks.ctypes.data, blocks.size, C.byref(io))
+
+
+def draw_trace(final: dict) -> np.ndarray:
+    """drawTrace (wavefront.cl:1835-1850): the RGBA32F image of the accumulators."""
+    n = min(final[k].size for k in ("final_r", "final_g", "final_b", "final_w"))
+    rgba = np.zeros((n, 4), np.float32)
+    lib().orc_draw_trace(n, *(np.ascontiguousarray(final[k], np.uint32).ctypes.data
+                             for k in ("final_r", "final_g", "final_b", "final_w")), rgba.ctypes.data)
+    return rgba

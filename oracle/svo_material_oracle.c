@@ -1,5 +1,7 @@
 /* CPU oracle of evaluateMaterial (wavefront.cl:1568-1739) with setupTrace (617-636),
- * traverseChunkNormal (500-561), normalVector (235-245) and saveAccumulator (338-355).
+ * traverseChunkNormal (500-561), normalVector (235-245) and saveAccumulator (338-355), and of
+ * the shading kernels evaluateBackground (1741-1797), resolveRemainders (1799-1833) and
+ * drawTrace (1835-1850).
  * TEST INFRASTRUCTURE ONLY.  The definitions are listed in svo_material_oracle.h. */
 #include "svo_material_oracle.h"
 
@@ -232,4 +234,97 @@ int orc_evaluate_material(const orc_trace_input *in, orc_material_io *io) {
     }
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* shading kernels of the kernel-by-kernel frame                             */
+/* ------------------------------------------------------------------------ */
+
+/* evaluateBackground, wavefront.cl:1741-1797 (the light mix at 1775-1788). */
+int orc_evaluate_background(uint8_t ambient, orc_shade_io *io) {
+  for (uint32_t i = 0; i < io->n; i++) {
+    const uint32_t id = io->queue[i];
+    if (id >= io->n_rays) continue;
+    const uint32_t parent = io->parents[id];
+    if (parent >= io->n_pixels) continue;
+    const float *dir = io->directions + 4 * (size_t)id;
+    const float dp = (dir[0] * 0.0f + dir[1] * 0.0f) + dir[2] * -1.0f; /* dot(dir, (0, 0, -1)) */
+    uint32_t c[3], lum;
+    if (dp > 0.8f) {
+      c[0] = c[1] = c[2] = 255u;
+      lum = to_uchar(dp * 200.0f);
+    } else {
+      c[0] = 135u;
+      c[1] = 206u;
+      c[2] = 235u;
+      lum = 1u;
+    }
+    const uint32_t w = io->weightings[id];
+    if (w < 255u) {
+      const uint32_t amb = ambient;
+      const uint32_t boundary = lum > amb ? 255u : 0u;
+      const uint32_t diff = lum > amb ? lum - amb : amb - lum; /* abs_diff */
+      const float q = (float)diff / ((float)(lum + amb) * 255.0f);
+      const uint8_t *base[3] = {io->colour_r, io->colour_g, io->colour_b};
+      for (int k = 0; k < 3; k++) {
+        const uint32_t b = base[k] ? base[k][id] : 0u;
+        const float v = (float)b + (q * (float)c[k]) * (float)(uint32_t)(boundary - b);
+        c[k] = to_uchar(v);
+      }
+    }
+    io->final_w[parent] += w;
+    if (lum > 0u) {
+      io->final_r[parent] += c[0] * w;
+      io->final_g[parent] += c[1] * w;
+      io->final_b[parent] += c[2] * w;
+    }
+  }
+  return 0;
+}
+
+/* resolveRemainders, wavefront.cl:1799-1833. */
+int orc_resolve_remainders(uint32_t base_depth, const orc_block *blocks, uint32_t n_blocks, orc_shade_io *io) {
+  const uint32_t B = base_depth;
+  for (uint32_t i = 0; i < io->n; i++) {
+    const uint32_t id = io->queue[i];
+    if (id >= io->n_rays) continue;
+    const uint32_t parent = io->parents[id];
+    if (parent >= io->n_pixels) continue;
+    const uint32_t w = io->weightings[id];
+    const uint64_t loc[3] = {io->locations[4 * (size_t)id], io->locations[4 * (size_t)id + 1],
+                             io->locations[4 * (size_t)id + 2]};
+    const uint32_t depth = io->depths[id];
+    if (depth < B + 2 || depth > MAX_DEPTH) continue;
+    /* getBlockAddress, wavefront.cl:296-304 */
+    uint32_t a = orc_base_location(B + 2, loc);
+    for (uint32_t d = B + 2; d < depth; d++) a = (a < n_blocks ? blocks[a].child : 0u) + orc_chunk_position(d, loc);
+    if (a == U32MAX) continue;
+    const orc_block zero = {0};
+    const orc_block blk = a < n_blocks ? blocks[a] : zero;
+    io->final_w[parent] += w;
+    io->final_r[parent] += (uint32_t)blk.r * w;
+    io->final_g[parent] += (uint32_t)blk.g * w;
+    io->final_b[parent] += (uint32_t)blk.b * w;
+  }
+  return 0;
+}
+
+/* drawTrace, wavefront.cl:1835-1850. */
+void orc_draw_trace(uint32_t n_pixels, const uint32_t *final_r, const uint32_t *final_g, const uint32_t *final_b,
+                    const uint32_t *final_w, float *rgba) {
+  for (uint32_t i = 0; i < n_pixels; i++) {
+    float *o = rgba + 4 * (size_t)i;
+    const uint32_t w = final_w[i];
+    if (w == 0u) {
+      o[0] = 0.75f;
+      o[1] = 0.0f;
+      o[2] = 0.0f;
+    } else {
+      const float fw = (float)w * 255.0f;
+      o[0] = fminf((float)final_r[i] / fw, 1.0f);
+      o[1] = fminf((float)final_g[i] / fw, 1.0f);
+      o[2] = fminf((float)final_b[i] / fw, 1.0f);
+    }
+    o[3] = 1.0f;
+  }
 }

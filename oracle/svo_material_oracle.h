@@ -1,5 +1,8 @@
 /*
- * svo_material_oracle.h -- CPU oracle of evaluateMaterial (wavefront.cl:1568-1739).
+ * svo_material_oracle.h -- CPU oracle of the second bounce: evaluateMaterial
+ * (wavefront.cl:1568-1739) and the kernel-by-kernel shading it feeds: evaluateBackground
+ * with the light mix of non-primary cones (1741-1797), resolveRemainders (1799-1833) and
+ * drawTrace (1835-1850).
  *
  * TEST INFRASTRUCTURE ONLY, like svo_oracle.h: used by tests/ as the checker and never
  * linked or loaded by the product.
@@ -58,6 +61,48 @@ typedef struct {
 
 /* evaluateMaterial over queue[0, n) with TraceInput `in` (BaseDepth, Tick, Fov*, Weighting*). */
 int orc_evaluate_material(const orc_trace_input *in, orc_material_io *io);
+
+/* The shading kernels over the per-trace SoA buffers of MainWindow (the product's
+ * kernel-by-kernel path): queue[0, n) of trace ids into the per-pixel accumulators of their
+ * parents.  Ids >= n_rays and parents >= n_pixels are skipped (the reference indexes past
+ * its buffers).  Accumulator sums wrap as u32 additions. */
+typedef struct {
+  const uint32_t *queue;
+  uint32_t n;
+  uint32_t n_rays;
+  const float *directions;   /* 4 floats per trace */
+  const uint32_t *parents;
+  const uint8_t *weightings;
+  const uint8_t *colour_r, *colour_g, *colour_b; /* may be NULL: read as 0 */
+  const uint64_t *locations; /* 4 u64 per trace (resolveRemainders) */
+  const uint8_t *depths;
+  uint32_t *final_r, *final_g, *final_b, *final_w;
+  uint32_t n_pixels;
+} orc_shade_io;
+
+/* evaluateBackground, wavefront.cl:1741-1797.  Sky (135, 206, 235) with luminosity 1, or the
+ * sun (255, 255, 255) with luminosity (uchar)(200 d) where d = dot(direction, (0, 0, -1)) >
+ * 0.8.  A cone of weighting below 255 mixes its base colour towards the light:
+ *   boundary = luminosity > ambient ? 255 : 0                                  (u32)
+ *   q = |luminosity - ambient| / ((luminosity + ambient) * 255)                (float; the
+ *       reference's native_divide is a correctly rounded division here)
+ *   colour = base + (q * light) * (float)(u32)(boundary - base)                (the u32
+ *       subtraction wraps when base > boundary, as in the reference)
+ * and the float result is converted to 0..255 by truncation with saturation (NaN -> 0; the
+ * reference's conversion of an out-of-range float to uchar is undefined).  W[parent] +=
+ * weighting; C[parent] += colour * weighting when the luminosity is non-zero. */
+int orc_evaluate_background(uint8_t ambient, orc_shade_io *io);
+
+/* resolveRemainders, wavefront.cl:1799-1833: the block at (Location, Depth) by the
+ * root-restart walk (getBlockAddress; entries with Depth outside [B+2, 60] or an address of
+ * UINT_MAX are skipped, reads past Blocks give a zero block), then W[parent] += weighting,
+ * C[parent] += colour * weighting. */
+int orc_resolve_remainders(uint32_t base_depth, const orc_block *blocks, uint32_t n_blocks, orc_shade_io *io);
+
+/* drawTrace, wavefront.cl:1835-1850: W = 0 -> (0.75, 0, 0, 1), else min(C / (W * 255), 1)
+ * per channel and alpha 1, into rgba (4 floats per pixel). */
+void orc_draw_trace(uint32_t n_pixels, const uint32_t *final_r, const uint32_t *final_g, const uint32_t *final_b,
+                    const uint32_t *final_w, float *rgba);
 
 /* The defined sin/cos above (exposed for tests). */
 void orc_sincos(float x, float *s, float *c);

@@ -72,13 +72,6 @@ __device__ __forceinline__ V3 normal_vector(int16_t pitch, int16_t yaw) {
   return {cy * cp, sy * cp, sp};
 }
 
-// (uchar)f: truncation, clamped to [0, 255]; NaN -> 0
-__device__ __forceinline__ uint32_t to_uchar(float f) {
-  if (!(f > 0.0f)) return 0u;
-  if (f >= 255.0f) return 255u;
-  return (uint32_t)f;
-}
-
 // traverseChunkNormal, wavefront.cl:500-561: one cell of level depth+1 along the normal,
 // to the farthest boundary (ties x, then y).
 __device__ __forceinline__ void step_off_surface(uint32_t depth, V3 n, uint64_t &x, uint64_t &y, uint64_t &z) {

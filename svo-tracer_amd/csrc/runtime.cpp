@@ -62,6 +62,15 @@ int ensure_scratch(std::shared_ptr<DeviceBuffer> &b, size_t bytes) {
   return SVO_OK;
 }
 
+int context_scan_sums(Context *ctx, size_t bytes, uint32_t **out) {
+  *out = nullptr;
+  if (!bytes) return SVO_OK;
+  int rc = ensure_scratch(ctx->scan_sums, bytes);
+  if (rc) return rc;
+  *out = (uint32_t *)ctx->scan_sums->ptr;
+  return SVO_OK;
+}
+
 // Kernel parameter names in argument order, ComputeManager.cs:313-344.
 const std::vector<std::string> &kernel_params(int kernel) {
   static const std::vector<std::vector<std::string>> tables = {

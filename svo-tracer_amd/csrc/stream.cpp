@@ -76,7 +76,10 @@ struct Stream {
       return SVO_OK;
     }
     if ((rc = launch(launch_select_count(u, lo, hi, pred, counts, ctx->stream), "select count"))) return rc;
-    if ((rc = launch(launch_scan_u32(counts, counts, chunks, counters + kShTotal, ctx->stream), "scan"))) return rc;
+    uint32_t *sums = nullptr;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
+    if ((rc = launch(launch_scan_u32(counts, counts, chunks, counters + kShTotal, sums, ctx->stream), "scan")))
+      return rc;
     uint32_t total = 0;
     if ((rc = read_u32(counters + kShTotal, &total))) return rc;
     if (slot < 0 || !total) {
@@ -339,7 +342,9 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     const uint32_t chunks = (n + 1023u) / 1024u;
     if ((rc = st.grow(kSbCounts, ((size_t)chunks + kShSlots) * 4, &p))) return rc;
     st.counters = (uint32_t *)p + chunks;
-    if ((rc = st.launch(launch_rank(flags, n, (uint32_t *)p, rank, st.counters + kShTotal, s, list, max_bricks),
+    uint32_t *sums = nullptr;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
+    if ((rc = st.launch(launch_rank(flags, n, (uint32_t *)p, rank, st.counters + kShTotal, sums, s, list, max_bricks),
                         "rank")))
       return rc;
     if ((rc = st.read_u32(st.counters + kShTotal, &unique))) return rc;
@@ -425,7 +430,9 @@ struct Builder {
     if (rc) return rc;
     uint32_t *c = ptr<uint32_t>(counts);
     const uint32_t chunks = (n + 1023) / 1024;
-    if ((rc = launched(launch_rank(flags, n, c, ptr<uint32_t>(*out), c + chunks, s), "rank"))) return rc;
+    uint32_t *sums = nullptr;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
+    if ((rc = launched(launch_rank(flags, n, c, ptr<uint32_t>(*out), c + chunks, sums, s), "rank"))) return rc;
     *total = 0;
     return n ? read(c + chunks, total, 4) : SVO_OK;
   }
@@ -529,11 +536,13 @@ int build_tree_device(Context *ctx, const GeomView &g, uint32_t base_depth, uint
     return rc;
   uint32_t root_total[2] = {0, 0};
   if (lv[0].n_int) {
+    uint32_t *sums = nullptr;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(lv[0].n_int), &sums))) return rc;
     if ((rc = bd.launched(launch_scan_u32(bd.ptr<uint32_t>(lv[0].alloc), bd.ptr<uint32_t>(lv[0].rank), lv[0].n_int,
-                                          tot, bd.s),
+                                          tot, sums, bd.s),
                           "root ranks")) ||
         (rc = bd.launched(launch_scan_u32(bd.ptr<uint32_t>(lv[0].writes), bd.ptr<uint32_t>(lv[0].start),
-                                          lv[0].n_int, tot + 1, bd.s),
+                                          lv[0].n_int, tot + 1, sums, bd.s),
                           "root writes")))
       return rc;
     if ((rc = bd.read(tot, root_total, 8))) return rc;

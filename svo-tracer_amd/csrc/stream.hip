@@ -691,13 +691,13 @@ int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint
   return rc_of();
 }
 
-int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s,
-                uint32_t *list, uint32_t limit) {
+int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total,
+                uint32_t *span_sums, hipStream_t s, uint32_t *list, uint32_t limit) {
   const uint32_t chunks = (n + kSelChunk - 1) / kSelChunk;
   if (!chunks) return 0;
   hipLaunchKernelGGL(flag_count_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts);
   if (rc_of()) return -1;
-  if (launch_scan_u32(counts, counts, chunks, total, s)) return -1;
+  if (launch_scan_u32(counts, counts, chunks, total, span_sums, s)) return -1;
   hipLaunchKernelGGL(flag_rank_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts, rank, list, limit);
   return rc_of();
 }

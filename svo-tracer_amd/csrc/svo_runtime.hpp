@@ -37,6 +37,9 @@ constexpr size_t kSmallScratchBytes = 2048;
 int hip_fail(hipError_t e, const char *what);
 int new_buffer(size_t bytes, std::shared_ptr<DeviceBuffer> *out);
 int ensure_scratch(std::shared_ptr<DeviceBuffer> &b, size_t bytes);
+struct Context;
+// The context's span-sum scratch, grown to `bytes` (null when bytes is 0).
+int context_scan_sums(Context *ctx, size_t bytes, uint32_t **out);
 const std::vector<std::string> &kernel_params(int kernel);
 bool valid_buffer_name(int name);
 
@@ -115,6 +118,8 @@ struct Context {
   // scratch of evaluateMaterial: per-entry counts and first ids, chunk sums, queue flags
   std::shared_ptr<DeviceBuffer> material_scratch[5];
   uint32_t *stream_host = nullptr;
+  // span sums of the multi-work-group scans (tree management, streaming, device builds)
+  std::shared_ptr<DeviceBuffer> scan_sums;
   uint32_t *id_host = nullptr;  // pinned word for the queue-counter read-backs (svo_reset_id)
   // batched cameras: concatenated camera tables (cache key), camera records
   std::shared_ptr<DeviceBuffer> scratch_batch_lut;

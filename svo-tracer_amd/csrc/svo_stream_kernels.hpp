@@ -39,9 +39,10 @@ int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *ke
 int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb,
                   svo_grafting *records, uint4 *bricks, hipStream_t s);
 // rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.  With
-// a list, the flagged entries of rank below limit are listed in rank order.
-int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total, hipStream_t s,
-                uint32_t *list = nullptr, uint32_t limit = 0);
+// a list, the flagged entries of rank below limit are listed in rank order.  span_sums:
+// scratch of the scan over the counts (scan_span_sums_bytes of the count of chunks).
+int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total,
+                uint32_t *span_sums, hipStream_t s, uint32_t *list = nullptr, uint32_t limit = 0);
 // Groups of [lo, hi) that satisfy a predicate, in ascending order: per-chunk counts, then the
 // scatter of the list given the scanned counts.
 uint32_t select_chunks(uint32_t lo, uint32_t hi);

@@ -66,6 +66,7 @@ struct TreeScratch {
   uint32_t *claimed;             // bitmap over groups
   uint32_t claimed_words;
   uint32_t *chunk_counts;        // collection scan, one per chunk of groups
+  uint32_t *scan_sums;           // span sums of the collection scan (scan_span_sums_bytes)
   uint32_t *list;                // collected groups
   uint32_t list_capacity;
   uint32_t *size, *where, *need, *start;  // graft: per record
@@ -111,7 +112,11 @@ int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, uint32_t t
 int launch_graft_sizes(const svo_grafting *rec, uint32_t n, uint32_t n_gb, const TreeScratch &t, hipStream_t s);
 int launch_graft_structure(const svo_grafting *rec, uint32_t n, const svo_block *gb, uint32_t n_gb,
                            const TreeScratch &t, hipStream_t s);
-int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, hipStream_t s);
+// Exclusive scan of n u32; the sum goes to *total.  Above 2 x 16384 elements the scan needs
+// span_sums, device scratch of scan_span_sums_bytes(n) bytes owned by the caller.
+size_t scan_span_sums_bytes(uint32_t n);
+int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, uint32_t *span_sums,
+                    hipStream_t s);
 int launch_graft_winners(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
 int launch_graft_roots(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
 // total_need: the sum of need over the n records (the scan total of need into start).

@@ -116,6 +116,8 @@ struct Op {
     ts.claimed = (uint32_t *)p;
     if ((rc = grow(kTbChunks, ((size_t)pv.n_usages / kCollectChunk + 2) * 4, &p))) return rc;
     ts.chunk_counts = (uint32_t *)p;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(pv.n_usages / kCollectChunk + 1), &ts.scan_sums)))
+      return rc;
     return tree_fill_u32(ts.counters, 0, kTcSlots, ctx->stream) ? fail(SVO_ERR_HIP, "tree: fill") : SVO_OK;
   }
 
@@ -283,7 +285,10 @@ int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
   op.ts.start = (uint32_t *)p;
   // structure: sizes, where each graft's numbering lives, pre-order numbering, needs
   if ((rc = op.launch(launch_graft_sizes(rec, n, n_gb, op.ts, s), "graft sizes"))) return rc;
-  if ((rc = op.launch(launch_scan_u32(op.ts.size, op.ts.where, n, op.ts.counters + kTcTotal, s), "scan"))) return rc;
+  uint32_t *sums = nullptr;
+  if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(n), &sums))) return rc;
+  if ((rc = op.launch(launch_scan_u32(op.ts.size, op.ts.where, n, op.ts.counters + kTcTotal, sums, s), "scan")))
+    return rc;
   if ((rc = op.read(kTcTotal, 1))) return rc;
   const size_t numbering = (size_t)op.host[kTcTotal] + 1;
   if ((rc = op.grow(kTbNumber, numbering * 4, &p))) return rc;
@@ -291,7 +296,8 @@ int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
   if ((rc = op.grow(kTbParent, numbering * 4, &p))) return rc;
   op.ts.parent = (uint32_t *)p;
   if ((rc = op.launch(launch_graft_structure(rec, n, gb, n_gb, op.ts, s), "graft structure"))) return rc;
-  if ((rc = op.launch(launch_scan_u32(op.ts.need, op.ts.start, n, op.ts.counters + kTcTotal, s), "scan"))) return rc;
+  if ((rc = op.launch(launch_scan_u32(op.ts.need, op.ts.start, n, op.ts.counters + kTcTotal, sums, s), "scan")))
+    return rc;
   if ((rc = op.read(kTcTotal, 1))) return rc;
   const uint32_t total_need = op.host[kTcTotal];
   uint32_t got = 0;

@@ -7,7 +7,6 @@
 // target, the record with the highest index wins (a hash table keyed by (field, target)
 // holding the largest record index).  Collection ranks candidates in ascending group
 // order with an order-preserving count / scan / scatter, one pass per age threshold.
-#include <mutex>
 
 #include "svo_device.hpp"
 #include "svo_tree_kernels.hpp"
@@ -797,27 +796,20 @@ int tree_fill_u64(unsigned long long *p, unsigned long long value, size_t n, hip
   return launch_rc();
 }
 
-// Span sums of the multi-work-group scan: one array per device, allocated on the first
-// large scan and kept until the process ends.  Scans on one device are stream-ordered
-// (every caller submits on its context's stream), so one array serves them all.
-constexpr uint32_t kMaxSpans = (uint32_t)((1ull << 32) / kScanSpan);
-static uint32_t *scan_partials(int device) {
-  static std::mutex mu;
-  static uint32_t *bufs[64] = {};
-  if (device < 0 || device >= 64) return nullptr;
-  std::lock_guard<std::mutex> lock(mu);
-  if (!bufs[device] && hipMalloc((void **)&bufs[device], (size_t)kMaxSpans * 4) != hipSuccess) bufs[device] = nullptr;
-  return bufs[device];
+size_t scan_span_sums_bytes(uint32_t n) {
+  return n <= 2u * kScanSpan ? 0 : (size_t)(((uint64_t)n + kScanSpan - 1) / kScanSpan) * 4;
 }
 
-int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, hipStream_t s) {
+// Arrays above two spans take three launches: span sums (into the caller's span_sums, which
+// belongs to one context, so scans on different contexts never share it), a one-work-group
+// scan of the sums, span scans from their offsets.
+int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, uint32_t *span_sums,
+                    hipStream_t s) {
   if (n <= 2u * kScanSpan) {
     hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(kScanWg), 0, s, in, out, n, total);
     return launch_rc();
   }
-  int device = 0;
-  if (hipGetDevice(&device) != hipSuccess) return -1;
-  uint32_t *partial = scan_partials(device);
+  uint32_t *partial = span_sums;
   if (!partial) return -1;
   const uint32_t spans = (uint32_t)(((uint64_t)n + kScanSpan - 1) / kScanSpan);
   hipLaunchKernelGGL(scan_reduce_kernel, dim3(spans), dim3(kScanWg), 0, s, in, n, partial);
@@ -907,7 +899,7 @@ int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, 
   hipLaunchKernelGGL(collect_count_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset,
                      live_only ? 1u : 0u, t.claimed, t.chunk_counts);
   if (launch_rc()) return -1;
-  return launch_scan_u32(t.chunk_counts, t.chunk_counts, chunks, &t.counters[kTcTotal], s);
+  return launch_scan_u32(t.chunk_counts, t.chunk_counts, chunks, &t.counters[kTcTotal], t.scan_sums, s);
 }
 
 int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only, uint32_t from,

@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(kThreads) background_kernel(ShadeParams p) {
   uint32_t lum;
   if (dp > 0.8f) {
     c[0] = c[1] = c[2] = 255u;
-    lum = (uint32_t)(uint8_t)(dp * 200.0f);
+    lum = to_uchar(dp * 200.0f);
   } else {
     c[0] = 135u;
     c[1] = 206u;
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(kThreads) background_kernel(ShadeParams p) {
                               p.colour_b ? p.colour_b[id] : 0u};
     for (int k = 0; k < 3; k++) {
       const float v = (float)base[k] + lq * (float)c[k] * (float)(uint32_t)(boundary - base[k]);
-      c[k] = v >= 255.0f ? 255u : (v > 0.0f ? (uint32_t)v : 0u);
+      c[k] = to_uchar(v);
     }
   }
   atomicAdd(&p.final_w[parent], w);

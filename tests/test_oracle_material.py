@@ -173,3 +173,102 @@ def test_capacity_counts_but_drops(svo, orc):
     assert n == full  # the id counter counts every cone
     assert n_bg == int(np.count_nonzero(c.out["weightings"][:5] > 5))  # only stored cones are queued
     assert np.array_equal(c2.out["directions"], c.out["directions"][:20])
+
+
+def _spine_closed_form(M: float, c: float, k: int):
+    """Spine angle s_k and cone half-angle f_k of spine step k (k >= 1), wavefront.cl:1656-1660,
+    from the closed form of the recurrence rather than by iterating it.  With t_k = s_{k-1} +
+    f_{k-1} (t_1 = c): s_k = (1+M)/(1-M) t_k, f_k = (M t_k + c)/(1-M), and t_{k+1} = s_k + f_k =
+    ((1+2M) t_k + c)/(1-M), a linear recurrence solved as t_k = a^(k-1) (c + c/(3M)) - c/(3M)
+    with a = (1+2M)/(1-M); at M = 0 it is t_k = k c (so s_k = k c and f_k = c)."""
+    if M == 0.0:
+        t = k * c
+    else:
+        a = (1.0 + 2.0 * M) / (1.0 - M)
+        t = a ** (k - 1) * (c + c / (3.0 * M)) - c / (3.0 * M)
+    return (1.0 + M) / (1.0 - M) * t, (M * t + c) / (1.0 - M)
+
+
+def _turn(v, axis, angle):
+    """The rotation matrix of wavefront.cl:1663-1681 / 1697-1714 in closed form,
+    cos(t) I + sin(t) [a]x + (1 - cos(t)) a a^T, applied to v and normalised (float64).  The
+    reference builds it from the axis as given: the spine axis is a unit vector, the ring
+    axis is the reflection, whose length is that of the incoming direction."""
+    r = v * math.cos(angle) + np.cross(axis, v) * math.sin(angle) + axis * axis.dot(v) * (1.0 - math.cos(angle))
+    return r / np.linalg.norm(r)
+
+
+def _expected_cones(normal, d, M, c, WM, WC, base_w):
+    """The cones evaluateMaterial emits, derived geometrically in float64: the mirror
+    reflection (not normalised); then for spine step k the ring seed -- the reflection turned by the spine
+    angle s_k about cross(d, n) in the negative sense -- and ring pairs j = 1, 2, ... (while
+    j f_k < pi/2) turned about the reflection by -2 j f_k and +2 j f_k; a spine step or ring
+    pair whose first vector points below the surface ends its loop.  Weights: the spine
+    step's weighting s_k WM + WC, scaled by dot/f_k where a cone's dot with the normal is
+    below f_k (the second vector of a pair takes the first one's), times the base weighting.
+    Returns (direction, fov, weight, margin) per cone; margin is the closest any decision
+    came to its boundary."""
+    refl = d - 2.0 * d.dot(normal) * normal
+    axis = np.cross(d, normal)
+    axis /= np.linalg.norm(axis)
+    out, margins = [], []
+
+    def weight(w, dot, fov):
+        return (w * dot / fov if dot < fov else w) * base_w
+
+    dp = refl.dot(normal)
+    out.append((refl, c, weight(WC, dp, c)))
+    s_prev, k = 0.0, 0
+    while s_prev < math.pi and k < 256:
+        margins.append(math.pi - s_prev)
+        k += 1
+        s, f = _spine_closed_form(M, c, k)
+        w = s * WM + WC
+        seed = _turn(refl, axis, -s)
+        dp = seed.dot(normal)
+        margins.append(abs(dp))
+        if dp < 0.0:
+            break
+        out.append((seed, f, weight(w, dp, f)))
+        j = 1
+        while j * f < math.pi / 2:
+            margins.append(math.pi / 2 - j * f)
+            r1 = _turn(seed, refl, -2.0 * j * f)
+            dp = r1.dot(normal)
+            margins.append(abs(dp))
+            if dp < 0.0:
+                break
+            out.append((r1, f, weight(w, dp, f)))
+            out.append((_turn(seed, refl, 2.0 * j * f), f, weight(w, dp, f)))
+            j += 1
+        s_prev = s
+    return out, min(margins)
+
+
+@pytest.mark.parametrize("params", [
+    {"fov_multiplier": 0.0, "fov_constant": 0.3, "weighting_multiplier": 0.0, "weighting_constant": 0.5},
+    {"fov_multiplier": 0.35, "fov_constant": 0.12, "weighting_multiplier": -0.1, "weighting_constant": 0.5},
+    {"fov_multiplier": 0.1, "fov_constant": 0.17, "weighting_multiplier": 0.05, "weighting_constant": 0.4},
+])
+def test_spine_and_ring_schedule_closed_form(svo, orc, params):
+    """Every cone of the oracle's schedule against the geometric derivation above: the same
+    number of cones, each direction within 1e-4, each cone's FoV the closed-form f_k (a
+    spine step that used the new FoV where the reference uses the previous one, or a ring
+    angle of j f instead of 2 j f, moves cones by far more), and each weighting within one
+    unit of its truncation."""
+    normal_in, d = (0.3, -0.2, 0.93), np.array([0.6, 0.1, -0.79], np.float64)
+    c = Case(svo, normal=normal_in, direction=d)
+    n_cones, _ = c.run(svo, orc, **params)
+    normal = _normal_of(c)
+    d32 = d.astype(np.float32).astype(np.float64)
+    exp, margin = _expected_cones(normal, d32, params["fov_multiplier"], params["fov_constant"],
+                                  params["weighting_multiplier"], params["weighting_constant"], 255.0)
+    assert margin > 1e-3, "the case sits too close to a loop or dot-product boundary to be decided in float32"
+    assert n_cones == len(exp)
+    dirs = c.dirs(n_cones)
+    fovs = c.out["fovs"][:n_cones].astype(np.float64)
+    w = c.out["weightings"][:n_cones].astype(np.int64)
+    for i, (e_dir, e_fov, e_w) in enumerate(exp):
+        assert np.allclose(dirs[i], e_dir, atol=1e-4), (i, dirs[i], e_dir)
+        assert abs(fovs[i] - e_fov) <= 1e-5 * max(1.0, e_fov), (i, fovs[i], e_fov)
+        assert abs(int(w[i]) - min(255, max(0, int(math.floor(e_w))))) <= 1, (i, w[i], e_w)
