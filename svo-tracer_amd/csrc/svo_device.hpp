@@ -29,6 +29,14 @@ __device__ __forceinline__ uint64_t float_to_ulong(float x) {
   return saturate ? kU64Max : v;
 }
 
+// (uchar)f of the reference's float-to-uchar conversions: truncation, saturated to [0, 255]
+// (NaN -> 0) where the reference's out-of-range conversion is undefined.
+__device__ __forceinline__ uint32_t to_uchar(float f) {
+  if (!(f > 0.0f)) return 0u;
+  if (f >= 255.0f) return 255u;
+  return (uint32_t)f;
+}
+
 // ulongToFloat, wavefront.cl:209: exact scaling by 2^-64.
 __device__ __forceinline__ float ulong_to_float(uint64_t x) { return (float)x * 5.42101086242752217e-20f; }
 
@@ -292,6 +300,11 @@ __device__ __forceinline__ bool trace_into_volume(const float o[3], const float 
 // as an all-zero record (defined; the reference reads past its buffer).
 __device__ __forceinline__ uint4 load_block(const uint4 *__restrict__ blocks, uint32_t n, uint32_t a) {
   return a < n ? blocks[a] : make_uint4(0u, 0u, 0u, 0u);
+}
+// The first 8 bytes of a Block record (Child @0, Chunk @4, NormalPitch @6): what a
+// traversal step reads of a block.
+__device__ __forceinline__ uint2 load_node(const uint4 *__restrict__ blocks, uint32_t n, uint32_t a) {
+  return a < n ? reinterpret_cast<const uint2 *>(blocks)[2u * (size_t)a] : make_uint2(0u, 0u);
 }
 __device__ __forceinline__ uint32_t block_child(uint4 q) { return q.x; }
 __device__ __forceinline__ uint32_t block_chunk(uint4 q) { return q.y & 0xffffu; }
