@@ -271,6 +271,13 @@ int svo_set_arg_buffer(svo_context *ctx, int kernel_name, const char *param, int
 /* SetArg<T>(KernelName, string, T / T[]), ComputeManager.cs:80-92 (by-value structs such as
  * TraceInput / UpdateInputData, size checked against the natural layout). */
 int svo_set_arg_bytes(svo_context *ctx, int kernel_name, const char *param, const void *value, size_t bytes);
+/* SetLocalArg(KernelName, string, BufferName), ComputeManager.cs:101-107: a local-memory
+ * argument of the named buffer's size.  The kernel, the parameter name and the buffer are
+ * checked as SetArg checks them; no kernel of this surface declares a local-memory parameter
+ * (the reference's kernels keep their local arrays inside, wavefront.cl), so a valid call
+ * fails with SVO_ERR_INVALID_ARGUMENT, as clSetKernelArg rejects a local size for a global
+ * pointer argument. */
+int svo_set_local_arg(svo_context *ctx, int kernel_name, const char *param, int buffer_name);
 /* Enqueue(KernelName, nuint[] workSize), ComputeManager.cs:72-78: global size only.  Queue
  * appends are in work-item order.  EvaluateMaterial (wavefront.cl:1568-1739) hands out
  * accumulator ids from AccumulatorID in canonical order (material entries in queue order,
@@ -290,7 +297,7 @@ int svo_finish(svo_context *ctx);
 typedef struct {
   uint32_t flags;          /* SVO_FRAME_* */
   uint32_t max_tree_depth; /* deepest block level that can be reached (stack sizing), 0 = 24 */
-  uint32_t workgroup_size; /* 0 = tuned default; 64, 128, 256 or 512 */
+  uint32_t workgroup_size; /* 0 = tuned default; 64, 128, 256, 512 or 1024 */
   uint32_t refill_lanes;   /* lanes idle before a wave fetches new cones, 0 = 64 (whole wave) */
   uint32_t stack_levels;   /* LDS short-stack levels, 0 = from max_tree_depth */
   uint32_t work_chunk;     /* cones a wave claims per atomic (rounded up to 64), 0 = 64 (one 8x8 tile) */
@@ -340,7 +347,7 @@ int svo_frame_graph_launch(svo_context *ctx, svo_frame_graph *graph, const svo_t
 int svo_frame_graph_destroy(svo_context *ctx, svo_frame_graph *graph);
 
 /* Picks svo_frame_options for a (batched) frame by timing `frames` real frames per
- * candidate (coordinate descent over workgroup_size 64..512, work_chunk 64..1024 cones per
+ * candidate (coordinate descent over workgroup_size 64..1024, work_chunk 64..1024 cones per
  * claim, refill_lanes 16..64 and stack_levels 4..24), starting from *base (may be NULL).  All candidates give
  * the same results, and repeating a frame leaves the same state, so the caller's frame can
  * be tuned in place.  No reference counterpart. */
@@ -501,6 +508,17 @@ typedef struct {
 
 int svo_tree_build(const svo_geometry *geometry, uint32_t base_depth, uint32_t max_depth, uint32_t max_size,
                    svo_tree **out);
+
+/* The builder's geometry on the host, for MakeBlock of arbitrary cells (TreeBuilder.MakeBlock /
+ * MakeChunk, TreeBuilder.cs:74-93, 264-284: the block the builder writes for that cell, with
+ * Child = UINT32_MAX).  cells holds n (level, x, y, z) quadruples, level <= 30; out receives n
+ * blocks.  No reference counterpart: the host side of on-demand refinement (the bricks the
+ * device generator makes in svo_stream_update equal these blocks), e.g. for CPU tools and
+ * checks.  A heightfield builds its full grid and min/max pyramid once, in svo_geometry_create. */
+typedef struct svo_geometry_host svo_geometry_host;
+int svo_geometry_create(const svo_geometry *geometry, svo_geometry_host **out);
+int svo_geometry_destroy(svo_geometry_host *geometry);
+int svo_geometry_blocks(const svo_geometry_host *geometry, const uint32_t *cells, uint32_t n, svo_block *out);
 int svo_tree_info_get(const svo_tree *tree, svo_tree_info *info);
 int svo_tree_copy_bases(const svo_tree *tree, uint16_t *dst, size_t count);
 int svo_tree_copy_blocks(const svo_tree *tree, svo_block *dst, size_t count);

@@ -357,11 +357,15 @@ _PROTOS = {
     "svo_swap": [C.c_void_p, C.c_int, C.c_int],
     "svo_buffer_device_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)],
     "svo_set_arg_buffer": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
+    "svo_set_local_arg": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
     "svo_set_arg_bytes": [C.c_void_p, C.c_int, C.c_char_p, C.c_void_p, C.c_size_t],
     "svo_enqueue": [C.c_void_p, C.c_int, C.c_uint32, C.POINTER(C.c_size_t)],
     "svo_flush": [C.c_void_p],
     "svo_finish": [C.c_void_p],
     "svo_trace_frame": [C.c_void_p, C.POINTER(TraceInput), C.POINTER(FrameOptions)],
+    "svo_geometry_create": [C.c_void_p, C.c_void_p],
+    "svo_geometry_destroy": [C.c_void_p],
+    "svo_geometry_blocks": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
     "svo_frame_stats_get": [C.c_void_p, C.POINTER(FrameStats)],
     "svo_trace_batch": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(FrameOptions)],
     "svo_frame_graph_create": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(FrameOptions),
@@ -555,6 +559,11 @@ class ComputeManager:
             b = _as_bytes(argument)
             rc = lib().svo_set_arg_bytes(self._h, int(kernel), param.encode(), b.ctypes.data, b.nbytes)
         _check(rc, f"SetKernelArg:{KernelName(kernel).name}:{param}")
+
+    def SetLocalArg(self, kernel: KernelName, param: str, buffer: BufferName) -> None:
+        """SetLocalArg(kernel, param, BufferName): a local-memory argument of the buffer's size."""
+        _check(lib().svo_set_local_arg(self._h, int(kernel), param.encode(), int(buffer)),
+               f"SetKernelArg:{KernelName(kernel).name}:{param}")
 
     def Enqueue(self, kernel: KernelName, work_size) -> None:
         ws = (C.c_size_t * len(work_size))(*[int(x) for x in work_size])
@@ -888,6 +897,32 @@ def build_tree(geometry: Geometry, base_depth: int, max_depth: int, max_size: in
     h = C.c_void_p()
     _check(lib().svo_tree_build(C.byref(geometry), base_depth, max_depth, max_size, C.byref(h)), "BuildTree")
     return Octree(h)
+
+
+class GeometryHost:
+    """The builder's geometry on the host (svo_geometry_create): MakeBlock of arbitrary cells."""
+
+    def __init__(self, geometry: Geometry):
+        self.handle = C.c_void_p()
+        _check(lib().svo_geometry_create(C.byref(geometry), C.byref(self.handle)), "GeometryCreate")
+
+    def blocks(self, cells: np.ndarray) -> np.ndarray:
+        """cells: (n, 4) uint32 rows of (level, x, y, z) -> n blocks (BLOCK_DTYPE)."""
+        c = np.ascontiguousarray(cells, np.uint32).reshape(-1, 4)
+        out = np.zeros(c.shape[0], BLOCK_DTYPE)
+        _check(lib().svo_geometry_blocks(self.handle, c.ctypes.data, c.shape[0], out.ctypes.data), "GeometryBlocks")
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().svo_geometry_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def camera_lut(trace_input: TraceInput):

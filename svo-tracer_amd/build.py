@@ -11,7 +11,6 @@ import concurrent.futures as cf
 import os
 import shutil
 import subprocess
-import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -66,16 +65,5 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
-def build_oracle(verbose: bool = False) -> Path:
-    res = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
-    if verbose:
-        print(f"built {ROOT / 'oracle' / 'liboracle.so'}")
-    return ROOT / "oracle" / "liboracle.so"
-
-
 if __name__ == "__main__":
     build(verbose=True)
-    if "--no-oracle" not in sys.argv:
-        build_oracle(verbose=True)

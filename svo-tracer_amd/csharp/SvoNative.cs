@@ -186,6 +186,7 @@ namespace SvoTracer.Kernel.Native
 
 		[DllImport(Lib)] public static extern int svo_set_arg_buffer(IntPtr ctx, int kernelName, [MarshalAs(UnmanagedType.LPStr)] string param, int bufferName);
 		[DllImport(Lib)] public static extern int svo_set_arg_bytes(IntPtr ctx, int kernelName, [MarshalAs(UnmanagedType.LPStr)] string param, void* value, nuint bytes);
+		[DllImport(Lib)] public static extern int svo_set_local_arg(IntPtr ctx, int kernelName, [MarshalAs(UnmanagedType.LPStr)] string param, int bufferName);
 		[DllImport(Lib)] public static extern int svo_enqueue(IntPtr ctx, int kernelName, uint dims, nuint* globalSize);
 		[DllImport(Lib)] public static extern int svo_flush(IntPtr ctx);
 		[DllImport(Lib)] public static extern int svo_finish(IntPtr ctx);
@@ -268,6 +269,12 @@ namespace SvoTracer.Kernel.Native
 		public void SetArg(KernelName kernelName, string paramName, BufferName bufferName)
 		{
 			HandleResultCode(SvoNative.svo_set_arg_buffer(_ctx, (int)kernelName, paramName, (int)bufferName),
+				$"SetKernelArg:{kernelName}:{paramName}");
+		}
+
+		public void SetLocalArg(KernelName kernelName, string paramName, BufferName bufferName)
+		{
+			HandleResultCode(SvoNative.svo_set_local_arg(_ctx, (int)kernelName, paramName, (int)bufferName),
 				$"SetKernelArg:{kernelName}:{paramName}");
 		}
 

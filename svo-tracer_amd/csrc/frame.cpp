@@ -74,6 +74,22 @@ int update_camera_table(Context *ctx, const svo_trace_input &in) {
   return SVO_OK;
 }
 
+int frame_scratch(FrameScratch &sc, std::shared_ptr<DeviceBuffer> &b, size_t bytes, void **out) {
+  if (sc.arena) {
+    const size_t at = (sc.arena_used + 255) & ~(size_t)255;
+    if (at > sc.arena_bytes || bytes > sc.arena_bytes - at)
+      return fail(SVO_ERR_CAPACITY, "trace frame: the frame needs more than the " + std::to_string(sc.arena_bytes) +
+                                        " bytes of its scratch region (" + std::to_string(at + bytes) + " so far)");
+    sc.arena_used = at + bytes;
+    *out = sc.arena + at;
+    return SVO_OK;
+  }
+  int rc = ensure_scratch(b, bytes);
+  if (rc) return rc;
+  *out = b->ptr;
+  return SVO_OK;
+}
+
 namespace {
 
 int check_size(const FrameBuffers::B &b, size_t need, const char *what) {
@@ -86,8 +102,8 @@ int check_size(const FrameBuffers::B &b, size_t need, const char *what) {
 int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, const svo_frame_options &o,
                   bool *bases_in_lds, uint32_t *stack_levels, LaunchShape *shape) {
   int wg = o.workgroup_size ? (int)o.workgroup_size : 256;
-  if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
-    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup_size must be 64, 128, 256 or 512");
+  if (wg != 64 && wg != 128 && wg != 256 && wg != 512 && wg != 1024)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup_size must be 64, 128, 256, 512 or 1024");
   uint32_t max_depth = o.max_tree_depth ? o.max_tree_depth : (ctx->tree_max_depth ? ctx->tree_max_depth : 24u);
   if (max_depth > 60) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: max_tree_depth must be <= 60");
   uint32_t levels = o.stack_levels ? o.stack_levels : (max_depth >= B + 2 ? max_depth - (B + 2) + 1 : 1u);
@@ -97,8 +113,11 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, 
   *bases_in_lds = bases_bytes <= 32 * 1024;
   if (!*bases_in_lds) bases_bytes = 0;
   bases_bytes += extra_lds;
-  while (levels > 1 && bases_bytes + (size_t)levels * wg * 4 > lds_limit) levels--;
-  size_t lds = bases_bytes + (size_t)levels * wg * 4;
+  // the short stack: per level and lane a block address, its Child field (u32 each) and its
+  // chunk (u16)
+  const auto stack_bytes = [&](uint32_t n) { return ((size_t)n * wg * kStackEntryBytes + 15) & ~(size_t)15; };
+  while (levels > 1 && bases_bytes + stack_bytes(levels) > lds_limit) levels--;
+  size_t lds = bases_bytes + stack_bytes(levels);
   if (lds > lds_limit) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup too large for LDS");
   // register budget: 5 waves per SIMD (96 VGPRs) keeps the step loop free of spills
   const int waves = o.waves_per_simd ? (int)o.waves_per_simd : 5;
@@ -218,16 +237,17 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
     if ((rc = check_size(fb.mat_count, 4, "MaterialQueueID"))) return rc;
   }
   plan->compaction = requests || request_id || queues;
+  void *sp = nullptr;
   if (plan->compaction && !p.status) {
-    if ((rc = ensure_scratch(sc.status, N))) return rc;
-    p.status = (uint8_t *)sc.status->ptr;
+    if ((rc = frame_scratch(sc, sc.status, N, &sp))) return rc;
+    p.status = (uint8_t *)sp;
   }
   if (requests || request_id) {
-    if ((rc = ensure_scratch(sc.requests, N * sizeof(svo_child_request)))) return rc;
-    p.req_sparse = (svo_child_request *)sc.requests->ptr;
+    if ((rc = frame_scratch(sc, sc.requests, N * sizeof(svo_child_request), &sp))) return rc;
+    p.req_sparse = (svo_child_request *)sp;
   }
-  if ((rc = ensure_scratch(sc.small, kSmallScratchBytes))) return rc;
-  char *small = (char *)sc.small->ptr;
+  if ((rc = frame_scratch(sc, sc.small, kSmallScratchBytes, &sp))) return rc;
+  char *small = (char *)sp;
   p.counters = (uint32_t *)(small + kCountersOffset);
   p.totals = (unsigned long long *)(small + kTotalsOffset);
 
@@ -235,11 +255,11 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   {
     std::vector<uint32_t> offsets;
     if (lut_dev) {
-      offsets.assign(lut_offsets, lut_offsets + n_cams);
       p.lut = lut_dev;
     } else {
       if ((rc = update_batch_tables(ctx, ins, n_cams, &offsets))) return rc;
       p.lut = (const float *)ctx->scratch_batch_lut->ptr;
+      lut_offsets = offsets.data();
     }
     uint64_t ray0 = 0;
     for (uint32_t c = 0; c < n_cams; c++) {
@@ -250,7 +270,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
       r.tiles_x = (r.W + 7) / 8;
       r.tile_begin = tiles;
       r.ray_begin = (uint32_t)ray0;
-      r.lut_offset = offsets[c];
+      r.lut_offset = lut_offsets[c];
       tiles += r.tiles_x * ((r.H + 7) / 8);
       ray0 += (uint64_t)r.W * r.H;
     }
@@ -267,7 +287,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   // defaults: a wave takes one 8x8 tile per claim and refills only when all its lanes are
   // idle -- spatial coherence of the 64 cones matters more than lane utilisation
   p.chunk = o.work_chunk ? ((std::min(o.work_chunk, 1u << 20) + 63u) / 64u) * 64u : 64u;
-  if (!(o.flags & SVO_FRAME_RASTER_ORDER) && tiles > 0) {
+  if (!(o.flags & SVO_FRAME_RASTER_ORDER) && tiles > 0 && !sc.arena) {
     const DeviceBuffer *old = sc.tile_cost.get();
     if ((rc = ensure_scratch(sc.tile_cost, (size_t)tiles * 4))) return rc;
     if ((rc = ensure_scratch(sc.tile_order, (size_t)tiles * 4))) return rc;
@@ -293,10 +313,10 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   std::memset(&c, 0, sizeof c);
   if (plan->compaction) {
     const uint32_t chunks = compaction_chunks((uint32_t)N);
-    if ((rc = ensure_scratch(sc.counts, (size_t)std::max(1u, chunks) * 16))) return rc;
+    if ((rc = frame_scratch(sc, sc.counts, (size_t)std::max(1u, chunks) * 16, &sp))) return rc;
     c.status = p.status;
     c.n = (uint32_t)N;
-    c.counts = (uint32_t *)sc.counts->ptr;
+    c.counts = (uint32_t *)sp;
     if (queues) {
       c.bg_queue = (uint32_t *)fb.bg_queue.ptr;
       c.mat_queue = (uint32_t *)fb.mat_queue.ptr;
@@ -402,11 +422,7 @@ static FrameBuffers::B named(Context *ctx, int name) {
   return b;
 }
 
-extern "C" int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options) {
-  if (!ctx || !input) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: null argument");
-  svo_frame_options o;
-  std::memset(&o, 0, sizeof o);
-  if (options) o = *options;
+FrameBuffers svo::frame_buffers(Context *ctx) {
   FrameBuffers fb;
   fb.bases = named(ctx, SVO_BUF_BASES);
   fb.blocks = named(ctx, SVO_BUF_BLOCKS);
@@ -424,6 +440,15 @@ extern "C" int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, c
   fb.mat_queue = named(ctx, SVO_BUF_MATERIAL_QUEUE);
   fb.bg_count = named(ctx, SVO_BUF_BACKGROUND_QUEUE_ID);
   fb.mat_count = named(ctx, SVO_BUF_MATERIAL_QUEUE_ID);
+  return fb;
+}
+
+extern "C" int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options) {
+  if (!ctx || !input) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_trace_frame: null argument");
+  svo_frame_options o;
+  std::memset(&o, 0, sizeof o);
+  if (options) o = *options;
+  const FrameBuffers fb = frame_buffers(ctx);
   return trace_frame_impl(ctx, *input, o, fb);
 }
 
@@ -433,22 +458,6 @@ extern "C" int svo_trace_batch(svo_context *ctx, const svo_trace_input *inputs, 
   svo_frame_options o;
   std::memset(&o, 0, sizeof o);
   if (options) o = *options;
-  FrameBuffers fb;
-  fb.bases = named(ctx, SVO_BUF_BASES);
-  fb.blocks = named(ctx, SVO_BUF_BLOCKS);
-  fb.usages = named(ctx, SVO_BUF_USAGES);
-  fb.requests = named(ctx, SVO_BUF_CHILD_REQUESTS);
-  fb.request_id = named(ctx, SVO_BUF_CHILD_REQUEST_ID);
-  fb.rgba = named(ctx, SVO_BUF_OUTPUT_IMAGE);
-  fb.hit = named(ctx, SVO_BUF_HIT_INDICES);
-  fb.visits = named(ctx, SVO_BUF_VISIT_COUNTS);
-  fb.hashes = named(ctx, SVO_BUF_VISIT_HASHES);
-  fb.status = named(ctx, SVO_BUF_RAY_STATUS);
-  fb.locations = named(ctx, SVO_BUF_LOCATIONS);
-  fb.depths = named(ctx, SVO_BUF_DEPTHS);
-  fb.bg_queue = named(ctx, SVO_BUF_BACKGROUND_QUEUE);
-  fb.mat_queue = named(ctx, SVO_BUF_MATERIAL_QUEUE);
-  fb.bg_count = named(ctx, SVO_BUF_BACKGROUND_QUEUE_ID);
-  fb.mat_count = named(ctx, SVO_BUF_MATERIAL_QUEUE_ID);
+  const FrameBuffers fb = frame_buffers(ctx);
   return trace_batch_impl(ctx, inputs, apertures, n_cameras, o, fb);
 }

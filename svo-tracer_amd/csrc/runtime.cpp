@@ -345,6 +345,19 @@ int svo_set_arg_buffer(svo_context *ctx, int kernel, const char *param, int name
   return SVO_OK;
 }
 
+int svo_set_local_arg(svo_context *ctx, int kernel, const char *param, int name) {
+  CHECK_CTX(ctx, "svo_set_local_arg");
+  if (kernel < 0 || kernel >= SVO_KERNEL_COUNT) return fail(SVO_ERR_UNKNOWN_KERNEL, "svo_set_local_arg: unknown kernel");
+  if (!param) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_set_local_arg: null parameter name");
+  const auto &names = kernel_params(kernel);
+  if (std::find(names.begin(), names.end(), param) == names.end())
+    return fail(SVO_ERR_UNKNOWN_PARAMETER, std::string("svo_set_local_arg: no parameter ") + param);
+  if (!ctx->get(name))
+    return fail(SVO_ERR_UNKNOWN_BUFFER, "svo_set_local_arg: buffer " + std::to_string(name) + " not initialised");
+  return fail(SVO_ERR_INVALID_ARGUMENT,
+              std::string("svo_set_local_arg: ") + param + " is not a local-memory argument of this kernel");
+}
+
 int svo_set_arg_bytes(svo_context *ctx, int kernel, const char *param, const void *value, size_t bytes) {
   CHECK_CTX(ctx, "svo_set_arg_bytes");
   if (kernel < 0 || kernel >= SVO_KERNEL_COUNT) return fail(SVO_ERR_UNKNOWN_KERNEL, "svo_set_arg_bytes: unknown kernel");

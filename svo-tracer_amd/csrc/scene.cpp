@@ -344,25 +344,68 @@ class Builder {
   }
 };
 
+// The builder geometry of a description (what TreeBuilder samples); `who` names the caller in
+// error messages.
+int make_geometry(const svo_geometry &d, const char *who, std::unique_ptr<Geometry> *out) {
+  try {
+    switch (d.kind) {
+      case SVO_GEOMETRY_SPHERES:
+        if (!d.shapes && d.count) return fail(SVO_ERR_INVALID_ARGUMENT, std::string(who) + ": no spheres");
+        out->reset(new Spheres(d.shapes, d.count));
+        return SVO_OK;
+      case SVO_GEOMETRY_CUBES:
+        if (!d.shapes && d.count) return fail(SVO_ERR_INVALID_ARGUMENT, std::string(who) + ": no cubes");
+        out->reset(new Cubes(d.shapes, d.count));
+        return SVO_OK;
+      case SVO_GEOMETRY_SPHERE_NOISE:
+        out->reset(new SphereNoise(d));
+        return SVO_OK;
+      case SVO_GEOMETRY_FBM_HEIGHTFIELD:
+        out->reset(new Heightfield(d));
+        return SVO_OK;
+      default:
+        return fail(SVO_ERR_INVALID_ARGUMENT, std::string(who) + ": unknown geometry kind");
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(SVO_ERR_OUT_OF_MEMORY, std::string(who) + ": out of host memory");
+  }
+}
+
 }  // namespace
 
 void camera_lut(const svo_trace_input &in, float *cos_u, float *sin_u, float *cos_v, float *sin_v) {
-  const float w = (float)in.screen_size[0], h = (float)in.screen_size[1];
-  for (uint32_t x = 0; x < in.screen_size[0]; x++) {
-    float u = in.fov[0] * ((w / 2.0f - (float)x) / w);
-    cos_u[x] = (float)std::cos((double)u);
-    sin_u[x] = (float)std::sin((double)u);
+  const uint32_t W = in.screen_size[0], H = in.screen_size[1];
+  for (uint32_t x = 0; x < W; x++) {
+    cos_u[x] = camera_lut_at(in, x);
+    sin_u[x] = camera_lut_at(in, (size_t)W + x);
   }
-  for (uint32_t y = 0; y < in.screen_size[1]; y++) {
-    float v = in.fov[1] * ((h / 2.0f - (float)y) / h);
-    cos_v[y] = (float)std::cos((double)v);
-    sin_v[y] = (float)std::sin((double)v);
+  for (uint32_t y = 0; y < H; y++) {
+    cos_v[y] = camera_lut_at(in, 2 * (size_t)W + y);
+    sin_v[y] = camera_lut_at(in, 2 * (size_t)W + H + y);
   }
+}
+
+float camera_lut_at(const svo_trace_input &in, size_t k) {
+  const uint32_t W = in.screen_size[0], H = in.screen_size[1];
+  const bool column = k < 2 * (size_t)W;
+  const size_t j = column ? k : k - 2 * (size_t)W;
+  const uint32_t n = column ? W : H;
+  const bool sine = j >= n;
+  const uint32_t i = (uint32_t)(sine ? j - n : j);
+  // u = FoV.x ((W/2 - x) / W), v = FoV.y ((H/2 - y) / H), setupInitialTrace (wavefront.cl:563-595)
+  const float size = (float)n;
+  const float t = in.fov[column ? 0 : 1] * ((size / 2.0f - (float)i) / size);
+  return (float)(sine ? std::sin((double)t) : std::cos((double)t));
 }
 
 }  // namespace svo
 
 using svo::fail;
+
+// A builder geometry kept for svo_geometry_blocks.
+struct svo_geometry_host {
+  std::unique_ptr<svo::Geometry> g;
+};
 
 extern "C" {
 
@@ -372,25 +415,9 @@ int svo_tree_build(const svo_geometry *geometry, uint32_t base_depth, uint32_t m
   if (base_depth < 1 || base_depth > 7) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: base_depth must be 1..7");
   if (max_depth > 20) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: max_depth must be <= 20");
   std::unique_ptr<svo::Geometry> g;
+  int grc = svo::make_geometry(*geometry, "svo_tree_build", &g);
+  if (grc) return grc;
   try {
-    switch (geometry->kind) {
-      case SVO_GEOMETRY_SPHERES:
-        if (!geometry->shapes && geometry->count) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: no spheres");
-        g.reset(new svo::Spheres(geometry->shapes, geometry->count));
-        break;
-      case SVO_GEOMETRY_CUBES:
-        if (!geometry->shapes && geometry->count) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: no cubes");
-        g.reset(new svo::Cubes(geometry->shapes, geometry->count));
-        break;
-      case SVO_GEOMETRY_SPHERE_NOISE:
-        g.reset(new svo::SphereNoise(*geometry));
-        break;
-      case SVO_GEOMETRY_FBM_HEIGHTFIELD:
-        g.reset(new svo::Heightfield(*geometry));
-        break;
-      default:
-        return fail(SVO_ERR_INVALID_ARGUMENT, "svo_tree_build: unknown geometry kind");
-    }
     std::unique_ptr<svo_tree> tree(new svo_tree());
     svo::Builder b(*g, tree->t, base_depth, max_depth);
     int rc = b.run(max_size);
@@ -400,6 +427,36 @@ int svo_tree_build(const svo_geometry *geometry, uint32_t base_depth, uint32_t m
   } catch (const std::bad_alloc &) {
     return fail(SVO_ERR_OUT_OF_MEMORY, "svo_tree_build: out of host memory");
   }
+}
+
+int svo_geometry_create(const svo_geometry *geometry, svo_geometry_host **out) {
+  if (!geometry || !out) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_geometry_create: null argument");
+  *out = nullptr;
+  std::unique_ptr<svo::Geometry> g;
+  int rc = svo::make_geometry(*geometry, "svo_geometry_create", &g);
+  if (rc) return rc;
+  svo_geometry_host *h = new (std::nothrow) svo_geometry_host();
+  if (!h) return fail(SVO_ERR_OUT_OF_MEMORY, "svo_geometry_create: out of host memory");
+  h->g = std::move(g);
+  *out = h;
+  return SVO_OK;
+}
+
+int svo_geometry_destroy(svo_geometry_host *geometry) {
+  delete geometry;
+  return SVO_OK;
+}
+
+int svo_geometry_blocks(const svo_geometry_host *geometry, const uint32_t *cells, uint32_t n, svo_block *out) {
+  if (!geometry || (n && (!cells || !out))) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_geometry_blocks: null argument");
+  svo::Tree scratch;
+  const svo::Builder b(*geometry->g, scratch, 1, 1);
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t level = cells[4 * (size_t)k];
+    if (level > 30) return fail(SVO_ERR_INVALID_ARGUMENT, "svo_geometry_blocks: level must be <= 30");
+    out[k] = b.make_block(svo::Cell{level, cells[4 * (size_t)k + 1], cells[4 * (size_t)k + 2], cells[4 * (size_t)k + 3]});
+  }
+  return SVO_OK;
 }
 
 int svo_tree_info_get(const svo_tree *tree, svo_tree_info *info) {

@@ -75,6 +75,8 @@ struct Tree {
 };
 
 void camera_lut(const svo_trace_input &in, float *cos_u, float *sin_u, float *cos_v, float *sin_v);
+// Entry k of the camera table laid out as cos_u[W], sin_u[W], cos_v[H], sin_v[H].
+float camera_lut_at(const svo_trace_input &in, size_t k);
 
 }  // namespace svo
 

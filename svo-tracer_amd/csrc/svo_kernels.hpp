@@ -11,6 +11,9 @@ namespace svo {
 
 // work counters sit on separate 128-byte lines (u32 stride)
 constexpr uint32_t kCounterStride = 32;
+// LDS bytes per short-stack entry of the trace kernel: block address and Child (u32 each),
+// chunk (u16)
+constexpr uint32_t kStackEntryBytes = 10;
 
 // One camera of a batched frame (96 bytes; read into LDS by the trace kernel).
 struct CamRec {

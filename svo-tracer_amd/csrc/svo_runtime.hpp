@@ -56,6 +56,8 @@ struct FrameBuffers {
 
 struct Context;
 int update_camera_table(Context *ctx, const svo_trace_input &in);
+// The context's named buffers a frame reads and writes.
+FrameBuffers frame_buffers(Context *ctx);
 // KernelName.Prune / KernelName.Graft behind svo_enqueue (arguments bound with SetArg).
 int enqueue_tree(Context *ctx, int kernel, uint32_t n);
 // Graft of device-resident records and graft blocks against the context's pool.
@@ -77,6 +79,10 @@ struct BatchPlan {
 // (a graph keeps its own so later frames never move memory a replay writes to).
 struct FrameScratch {
   std::shared_ptr<DeviceBuffer> status, requests, small, counts, cams;
+  // With an arena (device memory the caller owns), the frame's scratch is carved from it in
+  // 256-byte aligned pieces instead of being allocated, and the tile schedule is not kept.
+  uint8_t *arena = nullptr;
+  size_t arena_bytes = 0, arena_used = 0;
   // tile schedule: the per-tile cost of the last frame and the order built from it, valid
   // for the tiling (band ranges) they were measured on
   std::shared_ptr<DeviceBuffer> tile_cost, tile_order;
@@ -86,6 +92,9 @@ struct FrameScratch {
 // n_inline of host_cams (n_inline <= kInlineCams) to the device records, and orders the
 // tiles when the plan has a tile order.
 int launch_begin(const BatchPlan &plan, const CamRec *host_cams, uint32_t n_inline, hipStream_t stream);
+// `bytes` of device scratch for a frame: grown in b, or carved from sc's arena when it has one
+// (SVO_ERR_CAPACITY when the arena cannot hold it).
+int frame_scratch(FrameScratch &sc, std::shared_ptr<DeviceBuffer> &b, size_t bytes, void **out);
 int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const float *apertures, uint32_t n_cams,
                const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
                const float *lut_dev, const uint32_t *lut_offsets, BatchPlan *plan);

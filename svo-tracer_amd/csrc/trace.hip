@@ -93,7 +93,9 @@ trace_frame_kernel(const FrameParams p) {
   // moves up: every axis then moves towards 0); loc(a) gives the location itself
   uint64_t cx = 0, cy = 0, cz = 0, hash = kFnvOffset;
   float dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
-  uint4 node = make_uint4(0u, 0u, 0u, 0u);
+  uint32_t nchild = 0, nchunk = 0;  // Child field and chunk of the current block (node_addr)
+  float cone = 0.f;                // cone footprint at the current location, once cone_ok
+  bool cone_ok = false;
   uint32_t tot_vis = 0, tot_blk = 0;
   // updateUsage is resolved one step late so its load never stalls the step; within a
   // frame every write stores the frame tick, so the deferral does not change the result
@@ -104,23 +106,45 @@ trace_frame_kernel(const FrameParams p) {
     pend_g = kU32Max;
   };
 
-  auto stack_put = [&](uint32_t level, uint32_t a) {
+  // The short stack holds, for each block level from B+2 down, the block the cone passed
+  // through: its address, its Child field and its chunk (three [level][lane] arrays, so a
+  // pop to a stacked level needs no memory access).
+  uint32_t *st_child = s_stack + S * WG;
+  uint16_t *st_chunk = reinterpret_cast<uint16_t *>(s_stack + 2u * S * WG);
+  auto stack_put = [&](uint32_t level) {
     const uint32_t rel = level - (B + 2u);
-    if (rel < S) s_stack[rel * WG + tid] = a;
+    if (rel < S) {
+      s_stack[rel * WG + tid] = node_addr;
+      st_child[rel * WG + tid] = nchild;
+      st_chunk[rel * WG + tid] = (uint16_t)nchunk;
+    }
   };
-  auto stack_get = [&](uint32_t level) -> uint32_t {
+  // The block of `level` (a level the cone is still inside): address, Child field, chunk.
+  auto stack_get = [&](uint32_t level) -> uint3 {
     const uint32_t rel = level - (B + 2u);
-    if (rel < S) return s_stack[rel * WG + tid];
+    if (rel < S) return make_uint3(s_stack[rel * WG + tid], st_child[rel * WG + tid], st_chunk[rel * WG + tid]);
     // deeper than the stack: walk down from its last level along the current location
-    uint32_t a = s_stack[(S - 1u) * WG + tid];
-    for (uint32_t L = B + 2u + S - 1u; L < level; L++)
-      a = block_child(load_block(p.blocks, p.n_blocks, a)) + (chunk_position(L, cx, cy, cz) ^ signs);
-    return a;
+    uint32_t a = 0, ch = st_child[(S - 1u) * WG + tid], ck = 0;
+    for (uint32_t L = B + 2u + S - 1u; L < level; L++) {
+      a = ch + (chunk_position(L, cx, cy, cz) ^ signs);
+      const uint2 q = load_node(p.blocks, p.n_blocks, a);
+      ch = q.x;
+      ck = q.y & 0xffffu;
+    }
+    return make_uint3(a, ch, ck);
+  };
+  // Descends into the block at node_addr: fetches its Child field and chunk (one 64-bit
+  // load) and pushes it.
+  auto enter = [&](uint32_t level) {
+    const uint2 q = load_node(p.blocks, p.n_blocks, node_addr);
+    nchild = q.x;
+    nchunk = q.y & 0xffffu;
+    stack_put(level);
   };
 
   // Ends the lane's cone: evaluateBackground / resolveRemainders / drawTrace for a primary
   // cone (weighting 255, parent = itself), plus the per-cone extension outputs.
-  auto finish = [&](bool material, uint32_t addr, uint4 q, bool child, bool tree_size, bool capped) {
+  auto finish = [&](bool material, uint32_t addr, bool child, bool tree_size, bool capped) {
     float4 c;
     if (!material) {
       const float dp = (dx * 0.0f + dy * 0.0f) + dz * -1.0f;
@@ -139,6 +163,7 @@ trace_frame_kernel(const FrameParams p) {
     } else if (addr == kU32Max) {
       c = make_float4(0.75f, 0.0f, 0.0f, 1.0f);
     } else {
+      const uint4 q = load_block(p.blocks, p.n_blocks, addr);
       const float fw = 255.0f * 255.0f;
       c = make_float4(fminf((float)(block_r(q) * 255u) / fw, 1.0f), fminf((float)(block_g(q) * 255u) / fw, 1.0f),
                       fminf((float)(block_b(q) * 255u) / fw, 1.0f), 1.0f);
@@ -174,8 +199,7 @@ trace_frame_kernel(const FrameParams p) {
   };
 
   // A cone that ends only records how (`fin`: bit0 pending, bit1 material, bit2 child
-  // request, bit3 TreeSize, bit4 step cap; the block address in node_addr, the block in
-  // node); its outputs are written when the wave next refills or exits, so the shading
+  // request, bit3 TreeSize, bit4 step cap; the block address in node_addr); its outputs are written when the wave next refills or exits, so the shading
   // and output code runs once for all lanes that ended in between instead of once per
   // step in which any lane ended.
   uint32_t fin = 0;
@@ -186,7 +210,7 @@ trace_frame_kernel(const FrameParams p) {
   };
   auto flush = [&]() {
     if (fin) {
-      finish((fin & 2u) != 0u, node_addr, node, (fin & 4u) != 0u, (fin & 8u) != 0u, (fin & 16u) != 0u);
+      finish((fin & 2u) != 0u, node_addr, (fin & 4u) != 0u, (fin & 8u) != 0u, (fin & 16u) != 0u);
       fin = 0;
     }
   };
@@ -329,6 +353,7 @@ trace_frame_kernel(const FrameParams p) {
             nblk = 0;
             hash = kFnvOffset;
             last_group = kU32Max;
+            cone_ok = false;
             if (MODE != kModePlain) {
               cT = 1.0f;
               cR = cG = cB = 0.0f;
@@ -365,8 +390,9 @@ trace_frame_kernel(const FrameParams p) {
     while (active) {
       age++;
       // ---- one step of the reference state machine ----
-      // A step that moves to another block only sets node_addr and need_load; the block is
-      // loaded once, at the end of the step.
+      // A step that descends only sets node_addr and need_load; the block's Child field and
+      // chunk are loaded once, at the end of the step.  Steps that go back up take the
+      // block from the short stack, without a memory access.
       bool need_load = false;
       if (nvis >= kStepCap) {
         depth = 0;
@@ -384,11 +410,18 @@ trace_frame_kernel(const FrameParams p) {
                          yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
                          zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
           index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
-          chunk = p.bases_in_lds ? (uint32_t)s_bases[index] : (uint32_t)p.bases[index];
+          if (p.bases_in_lds) {
+            chunk = s_bases[index];
+          } else {
+            // a 32-bit read of the chunk pair (Bases holds an even number of chunks), so that
+            // the two paths stay an LDS read and a global read
+            const uint32_t w = reinterpret_cast<const uint32_t *>(p.bases)[index >> 1];
+            chunk = (w >> (16u * (index & 1u))) & 0xffffu;
+          }
         } else {
           // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
           index = node_addr;
-          chunk = block_chunk(node);
+          chunk = nchunk;
           const uint32_t g = index >> 3;
           if (g != last_group) {
             last_group = g;
@@ -416,20 +449,23 @@ trace_frame_kernel(const FrameParams p) {
               node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
                                         (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
                                         (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
-              stack_put(B + 2u, node_addr);
               need_load = true;
             }
           } else {
-            // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538
-            const float cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
-                                       cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
-                                       camf(cam, 13), camf(cam, 14));
+            // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538;
+            // the footprint depends on the location only, so it is computed once per location
+            if (!cone_ok) {
+              cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
+                             cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
+                             camf(cam, 13), camf(cam, 14));
+              cone_ok = true;
+            }
             // the LOD rule's five outcomes: up to level B+2 and hit, up one level, request a
             // child (hit), down to the child, hit
             const bool up_top = cone > exp2_neg(B + 2u);
             const bool up_one = !up_top && cone > exp2_neg(d);
             const bool stay = !up_top && !up_one;
-            const bool no_child = block_child(node) == kU32Max;
+            const bool no_child = nchild == kU32Max;
             // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
             const bool child = stay && no_child;
             const bool tree_size = child && cone <= exp2_neg(d + 1u);
@@ -437,12 +473,14 @@ trace_frame_kernel(const FrameParams p) {
             const bool hit = up_top || (stay && !down);
             depth = up_top ? B + 2u : (up_one ? d - 1u : (down ? d + 1u : d));
             if (up_top || up_one) {
-              node_addr = stack_get(depth);
+              const uint3 q = stack_get(depth);
+              node_addr = q.x;
+              nchild = q.y;
+              nchunk = q.z;
             } else if (down) {
-              node_addr = block_child(node) + oct;
-              stack_put(depth, node_addr);
+              node_addr = nchild + oct;
+              need_load = true;
             }
-            need_load = up_top || up_one || down;
             if (hit) {
               if (MODE == kModePlain) {
                 retire(true, node_addr, child, tree_size, false);
@@ -462,6 +500,7 @@ trace_frame_kernel(const FrameParams p) {
           // advance to the next cell boundary, pop levels that were left
           const uint64_t px = cx, py = cy, pz = cz;
           traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
+          cone_ok = false;
           const bool leave = leaving_canon(cx, cy, cz);
           uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
           // a block level pops to the level it names, except that level B+1 (no blocks)
@@ -470,8 +509,10 @@ trace_frame_kernel(const FrameParams p) {
           nd = (dd > B && nd == B + 1u) ? B : nd;
           depth = leave ? 0u : nd;
           if (pop_block) {
-            node_addr = stack_get(nd);
-            need_load = true;
+            const uint3 q = stack_get(nd);
+            node_addr = q.x;
+            nchild = q.y;
+            nchunk = q.z;
           }
           if (leave) {
             if (MODE == kModePlain)
@@ -481,7 +522,8 @@ trace_frame_kernel(const FrameParams p) {
           }
         }
       }
-      if (need_load) node = load_block(p.blocks, p.n_blocks, node_addr);
+      // a descent: fetch the block's Child field and chunk and push it
+      if (need_load && active) enter(depth);
       const uint32_t running = (uint32_t)__popcll(__ballot(active));
       if (fetching && 64u - running >= p.refill) break;
     }
@@ -804,6 +846,7 @@ int launch_waves(const FrameParams &p, const LaunchShape &shape, hipStream_t str
     case 128: return launch_trace_wg<128, WAVES>(p, shape, stream);
     case 256: return launch_trace_wg<256, WAVES>(p, shape, stream);
     case 512: return launch_trace_wg<512, WAVES>(p, shape, stream);
+    case 1024: return launch_trace_wg<1024, WAVES>(p, shape, stream);
     default: return -1;
   }
 }
@@ -815,6 +858,7 @@ int occupancy_waves(int wg, size_t lds, int *blocks_per_cu) {
     case 128: return occupancy_wg<128, WAVES>(lds, blocks_per_cu);
     case 256: return occupancy_wg<256, WAVES>(lds, blocks_per_cu);
     case 512: return occupancy_wg<512, WAVES>(lds, blocks_per_cu);
+    case 1024: return occupancy_wg<1024, WAVES>(lds, blocks_per_cu);
     default: return -1;
   }
 }

@@ -1,6 +1,6 @@
 // svo_autotune: picks the launch options of a (batched) frame by timing real frames.
 //
-// Coordinate descent over the work-group size (64..512 lanes), the work chunk a wave claims
+// Coordinate descent over the work-group size (64..1024 lanes), the work chunk a wave claims
 // per atomic (64..1024 cones: the queue chunk), the refill threshold (lanes idle before a
 // wave takes new cones) and the depth of the LDS short stack (levels kept per lane; below it
 // a pop walks down from the deepest kept level).  Every option gives bit-identical results, and re-running a frame
@@ -105,7 +105,7 @@ extern "C" int svo_autotune(svo_context *ctx, const svo_trace_input *inputs, con
     }
     return SVO_OK;
   };
-  if ((rc = try_field(&svo_frame_options::workgroup_size, {64, 128, 256, 512}))) return rc;
+  if ((rc = try_field(&svo_frame_options::workgroup_size, {64, 128, 256, 512, 1024}))) return rc;
   if ((rc = try_field(&svo_frame_options::work_chunk, {64, 128, 256, 512, 1024}))) return rc;
   if ((rc = try_field(&svo_frame_options::refill_lanes, {16, 32, 48, 64}))) return rc;
   if ((rc = try_field(&svo_frame_options::stack_levels, {4, 6, 8, 12, 16, 24}))) return rc;

@@ -64,6 +64,7 @@ def test_errors_raise_with_method_and_code(svo):
     lib = svo.lib()
     assert lib.svo_set_arg_buffer(None, 0, b"bases", 0) == -1
     assert "null context" in svo.last_error()
+    assert lib.svo_set_local_arg(None, 0, b"Bases", 0) == -1
     assert lib.svo_buffer_init(None, 0, 16, None, 0) == -1
     assert lib.svo_trace_frame(None, None, None) == -1
     assert lib.svo_prune(None, None, None, 0, None, 0, None, 0, None) == -1

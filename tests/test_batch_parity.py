@@ -84,7 +84,7 @@ def scene(svo):
     return tree, bases, blocks, tree.initial_usages(tick=1)
 
 
-@pytest.mark.parametrize("wg", [64, 256, 512])
+@pytest.mark.parametrize("wg", [64, 256, 512, 1024])
 def test_batch_matches_oracle(svo, orc, gpu, scene, wg):
     tree, bases, blocks, usages = scene
     cams = _cameras(svo, [(64, 48), (33, 17), (96, 72), (8, 8), (1, 1), (130, 9)])
@@ -205,7 +205,7 @@ def test_autotune_keeps_results(svo, orc, gpu, scene):
     apertures = [1.0, 2.0]
     want = gpu_batch(svo, gpu, bases, blocks, usages.copy(), cams, apertures)
     best, ms = gpu.Autotune(cams, apertures, frames=2, flags=svo.FRAME_QUEUES)
-    assert ms > 0 and best["workgroup_size"] in (0, 64, 128, 256, 512)
+    assert ms > 0 and best["workgroup_size"] in (0, 64, 128, 256, 512, 1024)
     got_usages = gpu.ReadBuffer(svo.BufferName.Usages, svo.USAGE_DTYPE, usages.size)
     assert first_mismatch(got_usages, want["usages"]) is None
     opts = {k: v for k, v in best.items() if v and k != "flags"}

@@ -222,3 +222,23 @@ def test_enqueue_errors(svo, gpu):
     with pytest.raises(svo.SvoError) as e:
         gpu.SetArg(K.Init, "input", b"\x00" * 10)
     assert e.value.code == -1
+
+
+def test_set_local_arg_checks_like_set_arg(svo, gpu):
+    """SetLocalArg (ComputeManager.cs:101-107): unknown kernel, parameter or buffer fail as SetArg
+    does; a valid call fails because no kernel of the surface has a local-memory parameter."""
+    K, B = svo.KernelName, svo.BufferName
+    gpu.InitDeviceBuffer(B.Origins, 64, 16)
+    with pytest.raises(svo.SvoError) as e:
+        gpu.SetLocalArg(K.Init, "NoSuchParameter", B.Origins)
+    assert e.value.code == -6
+    try:
+        gpu.ReleaseBuffer(B.Pruning)
+    except svo.SvoError:
+        pass  # not initialised
+    with pytest.raises(svo.SvoError) as e:
+        gpu.SetLocalArg(K.Init, "Origins", B.Pruning)
+    assert e.value.code == -4
+    with pytest.raises(svo.SvoError) as e:
+        gpu.SetLocalArg(K.Init, "Origins", B.Origins)
+    assert e.value.code == -1 and "local-memory" in str(e.value) and "SetKernelArg:Init:Origins" in str(e.value)
