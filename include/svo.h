@@ -325,6 +325,20 @@ enum {
  * Queue contents are in ascending ray-id order. */
 int svo_trace_frame(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options);
 
+/* svo_trace_frame without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same inputs, outputs and results as svo_trace_frame,
+ * except that tiles are taken in raster order (no tile schedule is kept between frames).  All
+ * of the frame's device scratch -- work counters and step totals, the camera record and
+ * table, one status byte per cone, 40 bytes per cone of staged child requests when
+ * ChildRequests or ChildRequestID is bound, the compaction counts -- comes from one static
+ * device array of 96 MiB per GPU (enough for a 1920x1080 frame with child requests and
+ * queues), held from the first call on that GPU until the process ends; a frame that needs
+ * more fails with SVO_ERR_CAPACITY before anything is launched.  The camera table is
+ * uploaded from host stack memory (one synchronisation).  Every call on a GPU shares that
+ * GPU's array, and calls that overlap on one GPU (several threads or contexts) race. */
+int svo_trace_frame_static(svo_context *ctx, const svo_trace_input *input, const svo_frame_options *options);
+
 /* Several cameras of uneven resolution in one persistent launch (no reference
  * counterpart; the batched form of svo_trace_frame).  The per-cone buffers hold the
  * cameras' cones concatenated in camera order (camera c's cone (x, y) at

@@ -366,6 +366,7 @@ _PROTOS = {
     "svo_geometry_create": [C.c_void_p, C.c_void_p],
     "svo_geometry_destroy": [C.c_void_p],
     "svo_geometry_blocks": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "svo_trace_frame_static": [C.c_void_p, C.POINTER(TraceInput), C.POINTER(FrameOptions)],
     "svo_frame_stats_get": [C.c_void_p, C.POINTER(FrameStats)],
     "svo_trace_batch": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(FrameOptions)],
     "svo_frame_graph_create": [C.c_void_p, C.POINTER(TraceInput), C.c_void_p, C.c_uint32, C.POINTER(FrameOptions),
@@ -708,6 +709,15 @@ class ComputeManager:
         o.work_chunk = work_chunk
         o.waves_per_simd = waves_per_simd
         _check(lib().svo_trace_frame(self._h, C.byref(trace_input), C.byref(o)), "TraceFrame")
+
+    def TraceFrameStatic(self, trace_input: TraceInput, flags: int = 0, **opts) -> None:
+        """svo_trace_frame_static: TraceFrame with its scratch in the library's static device
+        array (no allocation; raster tile order)."""
+        o = FrameOptions()
+        o.flags = flags
+        for k, v in opts.items():
+            setattr(o, k, v)
+        _check(lib().svo_trace_frame_static(self._h, C.byref(trace_input), C.byref(o)), "TraceFrameStatic")
 
     def TraceBatch(self, trace_inputs, apertures=None, flags: int = 0, max_tree_depth: int = 0,
                    workgroup_size: int = 0, refill_lanes: int = 0, stack_levels: int = 0, work_chunk: int = 0,

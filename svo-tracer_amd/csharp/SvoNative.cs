@@ -186,6 +186,7 @@ namespace SvoTracer.Kernel.Native
 
 		[DllImport(Lib)] public static extern int svo_set_arg_buffer(IntPtr ctx, int kernelName, [MarshalAs(UnmanagedType.LPStr)] string param, int bufferName);
 		[DllImport(Lib)] public static extern int svo_set_arg_bytes(IntPtr ctx, int kernelName, [MarshalAs(UnmanagedType.LPStr)] string param, void* value, nuint bytes);
+		[DllImport(Lib)] public static extern int svo_trace_frame_static(IntPtr ctx, ref NativeTraceInput input, ref NativeFrameOptions options);
 		[DllImport(Lib)] public static extern int svo_set_local_arg(IntPtr ctx, int kernelName, [MarshalAs(UnmanagedType.LPStr)] string param, int bufferName);
 		[DllImport(Lib)] public static extern int svo_enqueue(IntPtr ctx, int kernelName, uint dims, nuint* globalSize);
 		[DllImport(Lib)] public static extern int svo_flush(IntPtr ctx);
@@ -361,6 +362,10 @@ namespace SvoTracer.Kernel.Native
 		// ---- frame-level entry points (no ComputeManager counterpart) ----
 		public void TraceFrame(NativeTraceInput input, NativeFrameOptions options = default)
 			=> HandleResultCode(SvoNative.svo_trace_frame(_ctx, ref input, ref options), "TraceFrame");
+
+		// TraceFrame with its scratch in the library's static device array (no allocation).
+		public void TraceFrameStatic(NativeTraceInput input, NativeFrameOptions options = default)
+			=> HandleResultCode(SvoNative.svo_trace_frame_static(_ctx, ref input, ref options), "TraceFrameStatic");
 
 		public NativeTreeStats Prune(NativeUpdateInputData input, NativePruning[] records, NativeBlock[] surface, ulong[] locations)
 		{
