@@ -285,6 +285,57 @@ trace_frame_kernel(const FrameParams p) {
     active = false;
   };
 
+  // One step's visit (runBaseTrace / runBlockTrace up to the octant test, wavefront.cl:1400-1429,
+  // 1465-1505): counts the step, ticks the block's usage group (updateUsage, wavefront.cl:377-381)
+  // and tests the octant.  Returns 0 when it holds geometry (the step decides), 1 when it is
+  // empty (the step advances), 2 when the cone has reached the step cap and ended.
+  bool pend_adv = false;  // the current step was visited and advances
+  auto visit = [&](uint32_t &oct_out) -> int {
+    if (nvis >= kStepCap) {
+      depth = 0;
+      if (MODE == kModePlain)
+        retire(false, kU32Max, false, false, true);
+      else
+        end_composite();
+      return 2;
+    }
+    const uint32_t d = depth;
+    const bool is_block = d > B;
+    uint32_t index, chunk;
+    if (!is_block) {
+      const uint32_t xh = (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
+                     yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
+                     zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
+      index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
+      if (p.bases_in_lds) {
+        chunk = s_bases[index];
+      } else {
+        // a 32-bit read of the chunk pair (Bases holds an even number of chunks), so that
+        // the two paths stay an LDS read and a global read
+        const uint32_t w = reinterpret_cast<const uint32_t *>(p.bases)[index >> 1];
+        chunk = (w >> (16u * (index & 1u))) & 0xffffu;
+      }
+    } else {
+      index = node_addr;
+      chunk = nchunk;
+      const uint32_t g = index >> 3;
+      if (g != last_group) {
+        last_group = g;
+        resolve_usage();
+        if (g < p.n_usages) {
+          pend_g = g;
+          pend_t = p.usage16[4u * (size_t)g];
+        }
+      }
+    }
+    const uint32_t o = chunk_position(d, cx, cy, cz) ^ signs;
+    if (HASH) hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
+    nvis++;
+    nblk += is_block ? 1u : 0u;
+    oct_out = o;
+    return ((chunk >> (2u * o)) & 2u) == 0u ? 1 : 0;
+  };
+
   for (;;) {
     // ---- refill idle lanes with new cones (wave-uniform control flow) ----
     const uint64_t idle = __ballot(!active);
@@ -354,6 +405,7 @@ trace_frame_kernel(const FrameParams p) {
             hash = kFnvOffset;
             last_group = kU32Max;
             cone_ok = false;
+            pend_adv = false;
             if (MODE != kModePlain) {
               cT = 1.0f;
               cR = cG = cB = 0.0f;
@@ -387,143 +439,108 @@ trace_frame_kernel(const FrameParams p) {
     }
     // Step the running lanes until enough lanes are idle to refill: a lane leaves this loop
     // when its cone ends, so the exec mask keeps finished lanes' registers untouched.
+    // An iteration takes at most one advancing step and then one deciding step: a lane whose
+    // step advances visits its next step at once (the block it needs is in registers or on
+    // the short stack, never behind a load) and, when that step decides, takes it in the same
+    // iteration; a step visited as advancing waits as pend_adv for the next iteration.  Lanes
+    // then run the long advance and the decision together instead of in alternate iterations.
     while (active) {
       age++;
-      // ---- one step of the reference state machine ----
-      // A step that descends only sets node_addr and need_load; the block's Child field and
-      // chunk are loaded once, at the end of the step.  Steps that go back up take the
-      // block from the short stack, without a memory access.
-      bool need_load = false;
-      if (nvis >= kStepCap) {
-        depth = 0;
-        if (MODE == kModePlain)
-          retire(false, kU32Max, false, false, true);
-        else
-          end_composite();
-      } else {
+      bool adv = pend_adv, dec = false;
+      uint32_t oct = 0;
+      pend_adv = false;
+      if (!adv) {
+        const int v = visit(oct);
+        adv = v == 1;
+        dec = v == 0;
+      }
+      if (adv) {
+        // advance to the next cell boundary, pop levels that were left
+        const uint32_t dd = depth;
+        const uint64_t px = cx, py = cy, pz = cz;
+        traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
+        cone_ok = false;
+        const bool leave = leaving_canon(cx, cy, cz);
+        uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
+        // a block level pops to the level it names, except that level B+1 (no blocks)
+        // means the base level B; a pop to a shallower block level re-reads the stack
+        const bool pop_block = !leave && dd > B && nd > B + 1u && nd < dd;
+        nd = (dd > B && nd == B + 1u) ? B : nd;
+        depth = leave ? 0u : nd;
+        if (pop_block) {
+          const uint3 q = stack_get(nd);
+          node_addr = q.x;
+          nchild = q.y;
+          nchunk = q.z;
+        }
+        if (leave) {
+          if (MODE == kModePlain)
+            retire(false, kU32Max, false, false, false);
+          else
+            end_composite();
+        } else {
+          // the next step: visited now; an advancing one waits for the next iteration
+          const int v = visit(oct);
+          pend_adv = v == 1;
+          dec = v == 0;
+        }
+      }
+      if (dec) {
         const uint32_t d = depth;
-        const bool is_block = d > B;
-        uint32_t index, chunk;
-        if (!is_block) {
-          // runBaseTrace, wavefront.cl:1400-1463
-          const uint32_t xh = (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
-                         yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
-                         zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
-          index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
-          if (p.bases_in_lds) {
-            chunk = s_bases[index];
+        if (d <= B) {
+          // runBaseTrace's descent, wavefront.cl:1400-1463
+          if (d < B) {
+            depth = d + 1u;
           } else {
-            // a 32-bit read of the chunk pair (Bases holds an even number of chunks), so that
-            // the two paths stay an LDS read and a global read
-            const uint32_t w = reinterpret_cast<const uint32_t *>(p.bases)[index >> 1];
-            chunk = (w >> (16u * (index & 1u))) & 0xffffu;
+            depth = B + 2u;
+            node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
+                                      (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
+                                      (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
+            enter(depth);
           }
         } else {
-          // runBlockTrace, wavefront.cl:1465-1566; updateUsage, wavefront.cl:377-381
-          index = node_addr;
-          chunk = nchunk;
-          const uint32_t g = index >> 3;
-          if (g != last_group) {
-            last_group = g;
-            resolve_usage();
-            if (g < p.n_usages) {
-              pend_g = g;
-              pend_t = p.usage16[4u * (size_t)g];
-            }
+          // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538;
+          // the footprint depends on the location only, so it is computed once per location
+          if (!cone_ok) {
+            cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
+                           cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
+                           camf(cam, 13), camf(cam, 14));
+            cone_ok = true;
           }
-        }
-        const uint32_t oct = chunk_position(d, cx, cy, cz) ^ signs;
-        if (HASH) hash = (hash ^ (((uint64_t)d << 32) | index)) * kFnvPrime;
-        nvis++;
-        nblk += is_block ? 1u : 0u;
-
-        // empty octant, or (composite modes) a hit that the cone continues past
-        bool advance = ((chunk >> (2u * oct)) & 2u) == 0u;
-        uint32_t dd = d;  // the depth the advance leaves from
-        if (!advance) {
-          if (!is_block) {
-            if (d < B) {
-              depth = d + 1u;
-            } else {
-              depth = B + 2u;
-              node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
-                                        (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
-                                        (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
-              need_load = true;
-            }
-          } else {
-            // geometry in this octant: cone-footprint LOD decision, wavefront.cl:1516-1538;
-            // the footprint depends on the location only, so it is computed once per location
-            if (!cone_ok) {
-              cone = cone_at(camf(cam, 9), camf(cam, 10), camf(cam, 11), cx ^ sign_mask64(signs, 0u),
-                             cy ^ sign_mask64(signs, 1u), cz ^ sign_mask64(signs, 2u), camf(cam, 12),
-                             camf(cam, 13), camf(cam, 14));
-              cone_ok = true;
-            }
-            // the LOD rule's five outcomes: up to level B+2 and hit, up one level, request a
-            // child (hit), down to the child, hit
-            const bool up_top = cone > exp2_neg(B + 2u);
-            const bool up_one = !up_top && cone > exp2_neg(d);
-            const bool stay = !up_top && !up_one;
-            const bool no_child = nchild == kU32Max;
-            // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
-            const bool child = stay && no_child;
-            const bool tree_size = child && cone <= exp2_neg(d + 1u);
-            const bool down = stay && !no_child && cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth;
-            const bool hit = up_top || (stay && !down);
-            depth = up_top ? B + 2u : (up_one ? d - 1u : (down ? d + 1u : d));
-            if (up_top || up_one) {
-              const uint3 q = stack_get(depth);
-              node_addr = q.x;
-              nchild = q.y;
-              nchunk = q.z;
-            } else if (down) {
-              node_addr = nchild + oct;
-              need_load = true;
-            }
-            if (hit) {
-              if (MODE == kModePlain) {
-                retire(true, node_addr, child, tree_size, false);
-              } else {
-                composite(node_addr);
-                if (nlay >= p.max_layers) {
-                  end_composite();
-                } else {
-                  advance = true;
-                  dd = depth;
-                }
-              }
-            }
-          }
-        }
-        if (advance) {
-          // advance to the next cell boundary, pop levels that were left
-          const uint64_t px = cx, py = cy, pz = cz;
-          traverse_canon(dd, cx, cy, cz, dx, dy, dz, ix, iy, iz, signs);
-          cone_ok = false;
-          const bool leave = leaving_canon(cx, cy, cz);
-          uint32_t nd = compare_canon(dd, px, py, pz, cx, cy, cz);
-          // a block level pops to the level it names, except that level B+1 (no blocks)
-          // means the base level B; a pop to a shallower block level re-reads the stack
-          const bool pop_block = !leave && dd > B && nd > B + 1u && nd < dd;
-          nd = (dd > B && nd == B + 1u) ? B : nd;
-          depth = leave ? 0u : nd;
-          if (pop_block) {
-            const uint3 q = stack_get(nd);
+          // the LOD rule's five outcomes: up to level B+2 and hit, up one level, request a
+          // child (hit), down to the child, hit
+          const bool up_top = cone > exp2_neg(B + 2u);
+          const bool up_one = !up_top && cone > exp2_neg(d);
+          const bool stay = !up_top && !up_one;
+          const bool no_child = nchild == kU32Max;
+          // requestChild, wavefront.cl:362-375; TreeSize = min(1, (uchar)ConeDepth - Depth)
+          const bool child = stay && no_child;
+          const bool tree_size = child && cone <= exp2_neg(d + 1u);
+          const bool down = stay && !no_child && cone < exp2_neg(d + 1u) && d + 1u <= kMaxDepth;
+          const bool hit = up_top || (stay && !down);
+          depth = up_top ? B + 2u : (up_one ? d - 1u : (down ? d + 1u : d));
+          if (up_top || up_one) {
+            const uint3 q = stack_get(depth);
             node_addr = q.x;
             nchild = q.y;
             nchunk = q.z;
+          } else if (down) {
+            node_addr = nchild + oct;
+            enter(depth);
           }
-          if (leave) {
-            if (MODE == kModePlain)
-              retire(false, kU32Max, false, false, false);
-            else
-              end_composite();
+          if (hit) {
+            if (MODE == kModePlain) {
+              retire(true, node_addr, child, tree_size, false);
+            } else {
+              composite(node_addr);
+              if (nlay >= p.max_layers)
+                end_composite();
+              else
+                pend_adv = true;  // a composited layer: the cone continues past the cell
+            }
           }
         }
       }
-      // a descent: fetch the block's Child field and chunk and push it
-      if (need_load && active) enter(depth);
       const uint32_t running = (uint32_t)__popcll(__ballot(active));
       if (fetching && 64u - running >= p.refill) break;
     }
