@@ -94,6 +94,10 @@ trace_frame_kernel(const FrameParams p) {
   uint64_t cx = 0, cy = 0, cz = 0, hash = kFnvOffset;
   float dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
   uint32_t nchild = 0, nchunk = 0;  // Child field and chunk of the current block (node_addr)
+  // Morton code of the top min(depth, B) bits of the location: the level's Bases index is
+  // powSum(depth - 1) + bm, kept up to date as the cone descends and pops instead of being
+  // rebuilt from the location at every base step
+  uint32_t bm = 0;
   float cone = 0.f;                // cone footprint at the current location, once cone_ok
   bool cone_ok = false;
   uint32_t tot_vis = 0, tot_blk = 0;
@@ -303,10 +307,8 @@ trace_frame_kernel(const FrameParams p) {
     const bool is_block = d > B;
     uint32_t index, chunk;
     if (!is_block) {
-      const uint32_t xh = (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
-                     yh = (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
-                     zh = (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u);
-      index = base_offset(d) + (B <= 7u ? morton_top_base_hi(d, xh, yh, zh) : morton_top_hi(d, xh, yh, zh));
+      // powSum(d - 1) = 8 + 64 + ... + 8^(d-1): octal 1...10, i.e. bits 3, 6, ..., 3(d-1)
+      index = (0x249248u & ((1u << (3u * d)) - 1u)) + bm;
       if (p.bases_in_lds) {
         chunk = s_bases[index];
       } else {
@@ -419,6 +421,7 @@ trace_frame_kernel(const FrameParams p) {
             cx = ex ^ sign_mask64(signs, 0u);
             cy = ey ^ sign_mask64(signs, 1u);
             cz = ez ^ sign_mask64(signs, 2u);
+            bm = chunk_position(0u, ex, ey, ez);
             if (entered) {
               depth = 1;
               active = true;
@@ -473,6 +476,11 @@ trace_frame_kernel(const FrameParams p) {
           nchild = q.y;
           nchunk = q.z;
         }
+        // a pop to a base level keeps the top nd bits: drop the Morton triples below them.
+        // compare_canon never returns less than 1, so at nd == 1 the top bit itself may have
+        // changed (the cone crossed the volume's middle plane): rebuild it from the location.
+        if (nd <= B)
+          bm = nd == 1u ? (chunk_position(0u, cx, cy, cz) ^ signs) : bm >> (3u * ((dd < B ? dd : B) - nd));
         if (leave) {
           if (MODE == kModePlain)
             retire(false, kU32Max, false, false, false);
@@ -491,11 +499,11 @@ trace_frame_kernel(const FrameParams p) {
           // runBaseTrace's descent, wavefront.cl:1400-1463
           if (d < B) {
             depth = d + 1u;
+            bm = (bm << 3) | oct;
           } else {
+            // the dense level B+2: Morton code of the top B+2 bits (bm holds the top B)
             depth = B + 2u;
-            node_addr = morton_top_hi(B + 2u, (uint32_t)(cx >> 32) ^ sign_mask32(signs, 0u),
-                                      (uint32_t)(cy >> 32) ^ sign_mask32(signs, 1u),
-                                      (uint32_t)(cz >> 32) ^ sign_mask32(signs, 2u));
+            node_addr = (((bm << 3) | oct) << 3) | (chunk_position(B + 1u, cx, cy, cz) ^ signs);
             enter(depth);
           }
         } else {
