@@ -112,7 +112,8 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, 
   size_t bases_bytes = ((size_t)n_bases * 2 + 3) & ~(size_t)3;
   *bases_in_lds = bases_bytes <= 32 * 1024;
   if (!*bases_in_lds) bases_bytes = 0;
-  bases_bytes += extra_lds;
+  // camera records, then the table of ticked usage groups
+  bases_bytes += extra_lds + (size_t)kTickedSlots * 4;
   // the short stack: per level and lane a block address, its Child field (u32 each) and its
   // chunk (u16)
   const auto stack_bytes = [&](uint32_t n) { return ((size_t)n * wg * kStackEntryBytes + 15) & ~(size_t)15; };

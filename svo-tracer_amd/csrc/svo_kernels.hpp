@@ -14,6 +14,9 @@ constexpr uint32_t kCounterStride = 32;
 // LDS bytes per short-stack entry of the trace kernel: block address and Child (u32 each),
 // chunk (u16)
 constexpr uint32_t kStackEntryBytes = 10;
+// Slots of the trace kernel's per-workgroup LDS table of usage groups already ticked in the
+// launch (direct-mapped on the group index, u32 each)
+constexpr uint32_t kTickedSlots = 1024;
 
 // One camera of a batched frame (96 bytes; read into LDS by the trace kernel).
 struct CamRec {

@@ -53,7 +53,8 @@ trace_frame_kernel(const FrameParams p) {
   const uint32_t bases_words = p.bases_in_lds ? (p.n_bases + 1u) / 2u : 0u;
   uint32_t *s_cam = smem + bases_words;
   const uint32_t cam_words = p.n_cams * kCamWords;
-  uint32_t *s_stack = s_cam + cam_words;
+  uint32_t *s_ticked = s_cam + cam_words;
+  uint32_t *s_stack = s_ticked + kTickedSlots;
   if (p.bases_in_lds) {
     for (uint32_t i = threadIdx.x; i < p.n_bases; i += WG) s_bases[i] = p.bases[i];
   }
@@ -61,6 +62,7 @@ trace_frame_kernel(const FrameParams p) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(p.cams);
     for (uint32_t i = threadIdx.x; i < cam_words; i += WG) s_cam[i] = src[i];
   }
+  for (uint32_t i = threadIdx.x; i < kTickedSlots; i += WG) s_ticked[i] = kU32Max;
   __syncthreads();
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -101,8 +103,12 @@ trace_frame_kernel(const FrameParams p) {
   float cone = 0.f;                // cone footprint at the current location, once cone_ok
   bool cone_ok = false;
   uint32_t tot_vis = 0, tot_blk = 0;
-  // updateUsage is resolved one step late so its load never stalls the step; within a
-  // frame every write stores the frame tick, so the deferral does not change the result
+  // updateUsage is resolved one step late so its load never stalls the step.  Within a
+  // launch every write stores the same frame tick, so a group needs its tick read (and, if
+  // it is not 65534/65535 and differs, written) once: neither the deferral nor skipping a
+  // group already handled changes the result.  A group counts as handled when this cone
+  // visited it before (a block reached again by a pop) or when the workgroup's table of
+  // ticked groups (s_ticked, direct-mapped) holds it; the lane that put it there writes it.
   uint32_t pend_g = kU32Max, pend_t = 0;
   auto resolve_usage = [&]() {
     if (pend_g != kU32Max && pend_t < 65534u && pend_t != (tick & 0xffffu))
@@ -284,6 +290,7 @@ trace_frame_kernel(const FrameParams p) {
         }
       }
     }
+    resolve_usage();
     tot_vis += nvis;
     tot_blk += nblk;
     active = false;
@@ -325,8 +332,12 @@ trace_frame_kernel(const FrameParams p) {
         last_group = g;
         resolve_usage();
         if (g < p.n_usages) {
-          pend_g = g;
-          pend_t = p.usage16[4u * (size_t)g];
+          const uint32_t slot = g & (kTickedSlots - 1u);
+          if (s_ticked[slot] != g) {
+            s_ticked[slot] = g;
+            pend_g = g;
+            pend_t = p.usage16[4u * (size_t)g];
+          }
         }
       }
     }
@@ -475,6 +486,7 @@ trace_frame_kernel(const FrameParams p) {
           node_addr = q.x;
           nchild = q.y;
           nchunk = q.z;
+          last_group = q.x >> 3;  // ticked when the cone entered it
         }
         // a pop to a base level keeps the top nd bits: drop the Morton triples below them.
         // compare_canon never returns less than 1, so at nd == 1 the top bit itself may have
@@ -532,6 +544,7 @@ trace_frame_kernel(const FrameParams p) {
             node_addr = q.x;
             nchild = q.y;
             nchunk = q.z;
+            last_group = q.x >> 3;  // ticked when the cone entered it
           } else if (down) {
             node_addr = nchild + oct;
             enter(depth);
@@ -555,6 +568,7 @@ trace_frame_kernel(const FrameParams p) {
   }
 
   flush();
+  resolve_usage();  // a tick still pending (every cone ending resolves it; kept for safety)
   record_cost();
 
   // frame totals: one 64-bit atomic per wave
