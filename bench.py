@@ -6,11 +6,14 @@ the seeded 100-frame orbit.  Inputs (tree, usages, camera table) are resident
 in HBM before the timed region; the timed region is bracketed by stream
 synchronisation and measured with HIP events on the context's stream.
 
-Prints one JSON line; see DESIGN.md for the roofline model.
+Prints one JSON line.  Its roofline prices the trace kernel's algorithmic bytes
+(algorithmic_bytes below) against HBM, and its VALU instructions (from the committed
+counter profile) against the VALU issue rate, the roof the kernel presses.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -23,6 +26,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "svo-tracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue of MI355X: 256 CUs, each issuing one 64-lane VALU instruction per clock (four SIMDs,
+# a wave64 instruction every fourth clock each) at the 2.4 GHz peak engine clock
+VALU_PEAK_GINSTS = 256 * 2.4
+TRAFFIC_PROFILE = "profiles/trace_frame_1080p_d12_traffic.txt"
+COUNTER_PROFILE = "profiles/trace_frame_1080p_d12_counters.txt"
 METRIC = "Mcones/s @1080p depth-12 (plain vs batched); fwd+adjoint Mcones/s; HBM % peak"
 
 
@@ -37,14 +45,33 @@ def algorithmic_bytes(stats, cones: int, groups_touched: int) -> int:
     return 2 * stats.base_steps + 16 * stats.block_steps + 24 * cones + 8 * groups_touched
 
 
-def read_traffic(profile: Path):
-    """HBM bytes per trace-kernel dispatch from a committed rocprofv3 PMC summary, if present."""
+# the sources trace_frame_kernel is built and launched from: a counter profile describes this
+# build only when it records the same digest
+KERNEL_SOURCES = ("svo-tracer_amd/csrc/trace.hip", "svo-tracer_amd/csrc/svo_device.hpp",
+                  "svo-tracer_amd/csrc/svo_kernels.hpp", "svo-tracer_amd/csrc/frame.cpp")
+
+
+def kernel_sources_digest() -> str:
+    h = hashlib.sha1()
+    for f in KERNEL_SOURCES:
+        h.update((ROOT / f).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def read_profile_value(profile: Path, key: str, as_text: bool = False):
+    """A `key: value` line of a committed rocprofv3 summary, if present."""
     if not profile.exists():
         return None
     for line in profile.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_dispatch:"):
-            return float(line.split(":", 1)[1])
+        if line.startswith(key + ":"):
+            v = line.split(":", 1)[1].strip()
+            return v if as_text else float(v)
     return None
+
+
+def profile_matches_build(profile: Path) -> bool:
+    """True when the profile was taken of a kernel built from the current sources."""
+    return read_profile_value(profile, "kernel_sources_sha1", as_text=True) == kernel_sources_digest()
 
 
 def cpu_baseline(svo, tree, cams, seconds: float):
@@ -530,7 +557,8 @@ def main() -> int:
     cm.InitDeviceBuffer(B.VisitCounts, N, 4)
     opts = dict(workgroup_size=args.workgroup, refill_lanes=args.refill, waves_per_simd=args.waves,
                 work_chunk=args.chunk)
-    cams = [svo.scenes.config2_camera(f, W, H, frames=100, tick=f + 1) for f in range(K)]
+    # the timed frames are spread over the whole 100-frame orbit (poses f * 100 / K)
+    cams = [svo.scenes.config2_camera((f * 100) // K, W, H, frames=100, tick=f + 1) for f in range(K)]
 
     # untimed pass: exact per-frame work counts for the byte model
     alg = []
@@ -569,7 +597,13 @@ def main() -> int:
     kernel_s = kernel_ms / launches / 1e3
     bytes_per_launch = float(np.mean(alg))
     achieved = bytes_per_launch / kernel_s / 1e9
-    traffic = read_traffic(ROOT / "profiles" / "trace_frame_1080p_d12_traffic.txt")
+    # counter figures come from the committed profiles, and only when those were taken of this
+    # kernel build (same source digest); otherwise they are reported as null
+    traffic_fresh = profile_matches_build(ROOT / TRAFFIC_PROFILE)
+    counters_fresh = profile_matches_build(ROOT / COUNTER_PROFILE)
+    traffic = read_profile_value(ROOT / TRAFFIC_PROFILE, "traffic_bytes_per_dispatch") if traffic_fresh else None
+    valu_insts = read_profile_value(ROOT / COUNTER_PROFILE, "valu_insts_per_dispatch") if counters_fresh else None
+    valu_util = read_profile_value(ROOT / COUNTER_PROFILE, "valu_utilization_percent") if counters_fresh else None
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -585,7 +619,7 @@ def main() -> int:
         "data": "synthetic (seeded fBm heightfield scene, seeded orbit cameras)",
         "config": {
             "workload": f"plain forward, {W}x{H} primary cones, depth-{args.depth} fBm heightfield "
-                        f"({tree.block_count} x 16-B blocks), {K}-frame seeded orbit",
+                        f"({tree.block_count} x 16-B blocks), {K} frames spread over the 100-frame seeded orbit",
             "cones_per_step": N,
             "tree_blocks": tree.block_count,
         },
@@ -602,8 +636,25 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": f"{TRAFFIC_PROFILE}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch, scaled by "
+                              "the same counters over a device-to-device copy; kernel sources sha1 "
+                              f"{kernel_sources_digest()}, profile of this build: {traffic_fresh}",
         },
     }
+    if valu_insts:
+        # the roof the kernel actually presses: VALU issue (SQ_INSTS_VALU of the committed
+        # counter profile over this run's kernel time)
+        valu_achieved = valu_insts / kernel_s / 1e9
+        line["roofline"]["valu_issue"] = {
+            "achieved": round(valu_achieved, 1),
+            "peak": VALU_PEAK_GINSTS,
+            "unit": "G wave-instructions/s",
+            "frac": round(valu_achieved / VALU_PEAK_GINSTS, 4),
+            "insts_per_dispatch": valu_insts,
+            "lane_utilization_percent": valu_util,
+            "source": f"{COUNTER_PROFILE}: rocprofv3 --pmc SQ_INSTS_VALU, VALUUtilization per dispatch "
+                      "(profile of this build)",
+        }
     copy_gbs = copy_bandwidth(svo)
     line["hbm_copy"] = {"value": round(copy_gbs, 1), "unit": "GB/s",
                         "what": "2 GiB device-to-device copy, read + write bytes",
