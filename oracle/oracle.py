@@ -6,13 +6,15 @@ cpu_baseline leg of bench.py, as the checker.  The product never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 from typing import Optional
 
 import numpy as np
 
 ORACLE_DIR = Path(__file__).resolve().parent
-LIB_PATH = ORACLE_DIR / "liboracle.so"
+# SVO_ORACLE_LIB names another build of the oracle (the sanitized one of `make asan`)
+LIB_PATH = Path(os.environ["SVO_ORACLE_LIB"]) if os.environ.get("SVO_ORACLE_LIB") else ORACLE_DIR / "liboracle.so"
 
 U64MAX = (1 << 64) - 1
 U32MAX = (1 << 32) - 1

@@ -27,7 +27,8 @@ from typing import Optional, Union
 import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
-LIB_PATH = PKG_DIR / "libsvo.so"
+# SVO_LIB names another build of the library (the host-sanitized one of build.py --sanitize)
+LIB_PATH = Path(os.environ["SVO_LIB"]) if os.environ.get("SVO_LIB") else PKG_DIR / "libsvo.so"
 
 
 class KernelName(enum.IntEnum):
