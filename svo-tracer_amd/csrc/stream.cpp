@@ -4,8 +4,10 @@
 // requestChild, wavefront.cl:362-375), and the pool's garbage collection (reclaim of stale
 // groups, then compaction).
 //
-// Everything runs on the context's stream; the host reads a few totals back (request count,
-// distinct blocks, list sizes) to size the launches.
+// Everything runs on the context's stream.  The per-frame update sizes its launches from
+// device-side counts (the request count, the distinct blocks, the grafts' needs) and reads its
+// counters back once at the end, plus once after each collection pass that may be followed by
+// another; the garbage collection, which runs rarely, reads its list sizes back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +33,12 @@ struct svo_terrain {
 
 namespace {
 
-enum StreamBuf { kSbKeys, kSbVals, kSbFlags, kSbRank, kSbList, kSbCounts, kSbFrom, kSbTo, kSbRemap, kSbCount };
-enum : uint32_t { kShTotal = 0, kShSlots = 16 };
+enum StreamBuf { kSbFirst, kSbState, kSbFlags, kSbRank, kSbList, kSbCounts, kSbFrom, kSbTo, kSbRemap, kSbCount };
+// kShTotal: scan totals of select(); the update's own counters (kSbState): stored requests,
+// all requests, distinct requesting blocks, free groups
+enum : uint32_t { kShTotal = 0, kShStored = 0, kShRequests = 1, kShUnique = 2, kShFree = 3, kShSlots = 16 };
+// the pinned mirror holds the update's counters, then a tree-management counter block
+constexpr uint32_t kHostWords = kShSlots + kTcSlots;
 
 constexpr uint64_t kUsageInviolate = 0xFFFFFFFF0000FFFFull;  // tick 65535, parent UINT32_MAX
 constexpr uint64_t kUsageFree = 0xFFFFFFFF00000000ull;       // tick 0, parent UINT32_MAX
@@ -49,7 +55,7 @@ struct Stream {
   }
   int host_mirror() {
     if (ctx->stream_host) return SVO_OK;
-    hipError_t e = hipHostMalloc((void **)&ctx->stream_host, kShSlots * sizeof(uint32_t), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void **)&ctx->stream_host, kHostWords * sizeof(uint32_t), hipHostMallocDefault);
     return e == hipSuccess ? SVO_OK : hip_fail(e, "hipHostMalloc");
   }
   int read_u32(const uint32_t *dev, uint32_t *out) {
@@ -60,6 +66,20 @@ struct Stream {
     return SVO_OK;
   }
   int launch(int r, const char *what) { return r ? fail(SVO_ERR_HIP, std::string("stream: launch ") + what) : SVO_OK; }
+
+  // The number of groups of [lo, hi) satisfying `pred` into *total (device); nothing read back.
+  int count_into(const svo_usage *u, uint32_t lo, uint32_t hi, uint32_t pred, uint32_t *total) {
+    const uint32_t chunks = select_chunks(lo, hi);
+    void *p;
+    int rc = grow(kSbCounts, ((size_t)chunks + kShSlots) * 4, &p);
+    if (rc) return rc;
+    if (!chunks) return launch(tree_fill_u32(total, 0, 1, ctx->stream), "fill");
+    uint32_t *counts = (uint32_t *)p;
+    if ((rc = launch(launch_select_count(u, lo, hi, pred, counts, ctx->stream), "select count"))) return rc;
+    uint32_t *sums = nullptr;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
+    return launch(launch_scan_u32(counts, counts, chunks, total, sums, ctx->stream), "scan");
+  }
 
   // Ascending list of the groups of [lo, hi) satisfying `pred` into scratch slot `slot`;
   // returns its length.  counts / scan use the kSbCounts slot.
@@ -308,73 +328,87 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
   const PoolView pv = view_of(b, o->base_depth, o->tick);
   const uint32_t max_bricks = o->max_bricks ? o->max_bricks : 65536u;
 
-  // 1. the frame's child requests (the count keeps counting beyond what was stored)
-  uint32_t n = 0;
-  if (b.requests && b.request_id && b.request_id->bytes >= 4) {
-    uint32_t count = 0;
-    if ((rc = st.read_u32((const uint32_t *)b.request_id->ptr, &count))) return rc;
-    const uint32_t stored = (uint32_t)std::min<size_t>(b.requests->bytes / sizeof(svo_child_request), 0xffffffffu);
-    n = std::min(count, std::min(stored, o->max_child_request_id ? o->max_child_request_id : stored));
-    out.requests = count;
-  }
-  // 2. distinct requesting blocks that can take a child level, ranked in request order
-  uint32_t unique = 0, *flags = nullptr, *rank = nullptr, *list = nullptr;
-  if (n) {
-    uint32_t cap = 64;
-    while (cap < 2ull * n && cap < (1u << 31)) cap <<= 1;
-    void *p;
-    if ((rc = st.grow(kSbKeys, (size_t)cap * 4, &p))) return rc;
-    uint32_t *keys = (uint32_t *)p;
-    if ((rc = st.grow(kSbVals, (size_t)cap * 4, &p))) return rc;
-    uint32_t *vals = (uint32_t *)p;
-    if ((rc = st.grow(kSbFlags, (size_t)n * 4, &p))) return rc;
-    flags = (uint32_t *)p;
-    if ((rc = st.grow(kSbRank, (size_t)n * 4, &p))) return rc;
-    rank = (uint32_t *)p;
-    if ((rc = st.grow(kSbList, (size_t)std::max(1u, std::min(n, max_bricks)) * 4, &p))) return rc;
-    list = (uint32_t *)p;
-    if (tree_fill_u32(keys, 0xffffffffu, cap, s) || tree_fill_u32(vals, 0xffffffffu, cap, s))
-      return fail(SVO_ERR_HIP, "stream: fill");
-    if ((rc = st.launch(launch_request_unique((const svo_child_request *)b.requests->ptr, n, keys, vals, cap - 1u,
-                                              (const uint4 *)pv.blocks, pv.n_blocks, o->max_depth, flags, s),
-                        "request unique")))
-      return rc;
-    const uint32_t chunks = (n + 1023u) / 1024u;
-    if ((rc = st.grow(kSbCounts, ((size_t)chunks + kShSlots) * 4, &p))) return rc;
-    st.counters = (uint32_t *)p + chunks;
-    uint32_t *sums = nullptr;
-    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
-    if ((rc = st.launch(launch_rank(flags, n, (uint32_t *)p, rank, st.counters + kShTotal, sums, s, list, max_bricks),
-                        "rank")))
-      return rc;
-    if ((rc = st.read_u32(st.counters + kShTotal, &unique))) return rc;
-  }
-  out.unique = unique;
-  // 3. records and bricks into Grafting / GraftingBlocks, then the graft
-  const uint32_t nb = std::min(unique, max_bricks);
+  void *p;
+  if ((rc = st.grow(kSbState, kShSlots * 4, &p))) return rc;
+  uint32_t *state = (uint32_t *)p;
+  if ((rc = st.launch(tree_fill_u32(state, 0, kShSlots, s), "fill"))) return rc;
+  const uint32_t *tree_counters = nullptr;
   svo_update_input_data in;
   std::memset(&in, 0, sizeof in);
   in.base_depth = (uint8_t)o->base_depth;
   in.tick = o->tick;
   in.offset = o->prune_age;
-  in.graft_size = 8u * nb;
-  if (nb) {
+
+  // 1. the frame's child requests (the count keeps counting beyond what was stored): at most
+  // `stored`, the exact number on the device
+  uint32_t stored = 0;
+  if (b.requests && b.request_id && b.request_id->bytes >= 4) {
+    stored = (uint32_t)std::min<size_t>(b.requests->bytes / sizeof(svo_child_request), 0xffffffffu);
+    if (o->max_child_request_id) stored = std::min(stored, o->max_child_request_id);
+    if ((rc = st.launch(launch_request_count((const uint32_t *)b.request_id->ptr, stored, state, s), "request count")))
+      return rc;
+  }
+  // 2. distinct requesting blocks that can take a child level, ranked in request order
+  if (stored) {
+    const std::shared_ptr<DeviceBuffer> before = ctx->stream_scratch[kSbFirst];
+    if ((rc = st.grow(kSbFirst, (size_t)pv.n_blocks * 8, &p))) return rc;
+    unsigned long long *first = (unsigned long long *)p;
+    // a new table starts at generation 0, below every generation an update uses
+    if (ctx->stream_scratch[kSbFirst] != before || ++ctx->stream_gen == 0) {
+      if ((rc = st.launch(tree_fill_u64(first, 0ull, ctx->stream_scratch[kSbFirst]->bytes / 8, s), "fill"))) return rc;
+      ctx->stream_gen = 1;
+    }
+    if ((rc = st.grow(kSbFlags, (size_t)stored * 4, &p))) return rc;
+    uint32_t *flags = (uint32_t *)p;
+    if ((rc = st.grow(kSbRank, (size_t)stored * 4, &p))) return rc;
+    uint32_t *rank = (uint32_t *)p;
+    if ((rc = st.grow(kSbList, (size_t)std::max(1u, std::min(stored, max_bricks)) * 4, &p))) return rc;
+    uint32_t *list = (uint32_t *)p;
+    if ((rc = st.launch(launch_request_unique((const svo_child_request *)b.requests->ptr, stored, state + kShStored,
+                                              first, ctx->stream_gen, (const uint4 *)pv.blocks, pv.n_blocks,
+                                              o->max_depth, flags, s),
+                        "request unique")))
+      return rc;
+    const uint32_t chunks = (stored + 1023u) / 1024u;
+    if ((rc = st.grow(kSbCounts, ((size_t)chunks + kShSlots) * 4, &p))) return rc;
+    uint32_t *sums = nullptr;
+    if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
+    if ((rc = st.launch(launch_rank(flags, stored, (uint32_t *)p, rank, state + kShUnique, sums, s, list, max_bricks),
+                        "rank")))
+      return rc;
+    // 3. records and bricks of the first min(unique, max_bricks) into Grafting /
+    // GraftingBlocks (sized for max_bricks), then the graft
+    const uint32_t nb_max = std::min(stored, max_bricks);
+    in.graft_size = 8u * nb_max;
     std::shared_ptr<DeviceBuffer> rec, gb;
-    if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING, (size_t)nb * sizeof(svo_grafting), &rec))) return rc;
-    if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING_BLOCKS, (size_t)nb * 8 * sizeof(svo_block), &gb))) return rc;
-    if ((rc = st.launch(launch_bricks(t->view, (const svo_child_request *)b.requests->ptr, list, nb,
-                                      (svo_grafting *)rec->ptr, (uint4 *)gb->ptr, s),
+    if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING, (size_t)nb_max * sizeof(svo_grafting), &rec))) return rc;
+    if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING_BLOCKS, (size_t)nb_max * 8 * sizeof(svo_block), &gb))) return rc;
+    if ((rc = st.launch(launch_bricks(t->view, (const svo_child_request *)b.requests->ptr, list, nb_max,
+                                      state + kShUnique, (svo_grafting *)rec->ptr, (uint4 *)gb->ptr, s),
                         "bricks")))
       return rc;
-    svo_tree_stats ts;
-    if ((rc = graft_device(ctx, in, (const svo_grafting *)rec->ptr, nb, (const svo_block *)gb->ptr, 8u * nb, &ts)))
+    if ((rc = graft_device_counted(ctx, in, (const svo_grafting *)rec->ptr, nb_max, state + kShUnique,
+                                   (const svo_block *)gb->ptr, 8u * nb_max, &tree_counters)))
       return rc;
+  }
+  // the free groups, then every counter of the update in one read
+  if ((rc = st.count_into(pv.usages, 0, pv.n_usages, kSelFree, state + kShFree))) return rc;
+  uint32_t *host = ctx->stream_host;
+  hipError_t e = hipMemcpyAsync(host, state, kShSlots * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && tree_counters)
+    e = hipMemcpyAsync(host + kShSlots, tree_counters, kTcSlots * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "svo_stream_update: counter read");
+  out.requests = stored ? host[kShRequests] : 0u;
+  out.unique = host[kShUnique];
+  if (tree_counters) {
+    svo_tree_stats ts;
+    if ((rc = stats_from(host + kShSlots, &ts))) return rc;
     out.bricks = ts.placed;
     out.collected = ts.collected;
   }
+  uint32_t n_free = host[kShFree];
   // 4. garbage collection when free groups fall below gc_free_percent of the pool
-  uint32_t n_free = 0;
-  if ((rc = free_groups(ctx, pv, &n_free))) return rc;
   const bool want_gc = !(o->flags & SVO_STREAM_NO_GC) &&
                        ((o->flags & SVO_STREAM_FORCE_GC) ||
                         (uint64_t)n_free * 100u < (uint64_t)o->gc_free_percent * pv.n_usages);
@@ -386,8 +420,7 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
   }
   out.free_groups = n_free;
   out.pool_groups = pv.n_usages;
-  hipError_t e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(e, "svo_stream_update");
+  if (want_gc && (e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "svo_stream_update");
   if (stats) *stats = out;
   return SVO_OK;
 }

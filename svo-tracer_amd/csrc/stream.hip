@@ -174,75 +174,68 @@ __global__ __launch_bounds__(kWg) void seed_dense_kernel(TerrainView t, const ui
 // child requests -> bricks
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t addr_hash(uint32_t a, uint32_t mask) { return (a * 0x9E3779B1u) & mask; }
+// The frame's request count, capped at what the buffer stores: counters[0] = stored, [1] = all.
+__global__ void request_count_kernel(const uint32_t *request_id, uint32_t stored, uint32_t *counters) {
+  if (threadIdx.x != 0) return;
+  const uint32_t c = *request_id;
+  counters[0] = c < stored ? c : stored;
+  counters[1] = c;
+}
 
-// First occurrence of every requested address among the requests that ask for a level
-// (the smallest such request index).
-__global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_request *req, uint32_t n, uint32_t *keys,
-                                                            uint32_t *vals, uint32_t mask, uint32_t n_blocks) {
-  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-  if (i >= n) return;
+// A request takes part in the de-duplication when its address is in Blocks and its cone needs
+// a level below the block (TreeSize, wavefront.cl:362-375), and it is not right after a
+// taking-part request for the same address (neighbouring cones mostly ask for the same block).
+__device__ __forceinline__ bool request_inserts(const svo_child_request *req, uint32_t i, uint32_t n_blocks) {
   const uint32_t a = req[i].address;
-  // never keys: addresses out of range, and requests whose cone needs no level below the
-  // block (TreeSize 0, wavefront.cl:362-375); request_flag_kernel rejects them
-  if (a >= n_blocks || req[i].tree_size == 0) return;
-  // neighbouring cones mostly ask for the same block: a request right after an inserted
-  // request for the same address is not a first occurrence and needs no atomic
-  if (i > 0 && req[i - 1].address == a && req[i - 1].tree_size != 0) return;
-  uint32_t h = addr_hash(a, mask);
-  for (uint32_t probe = 0; probe <= mask; probe++) {
-    uint32_t k = keys[h];
-    if (k == kU32Max) k = atomicCAS(&keys[h], kU32Max, a);
-    if (k == kU32Max || k == a) {
-      atomicMin(&vals[h], i);
-      return;
-    }
-    h = (h + 1u) & mask;
-  }
+  if (a >= n_blocks || req[i].tree_size == 0) return false;
+  return !(i > 0 && req[i - 1].address == a && req[i - 1].tree_size != 0);
+}
+
+// first[a] = (gen << 32) | ~i for the smallest index i of the requests for block a in this
+// update (generation gen): a 64-bit maximum, so entries of earlier updates (smaller gen)
+// never need clearing.
+__device__ __forceinline__ unsigned long long first_key(uint32_t gen, uint32_t i) {
+  return ((unsigned long long)gen << 32) | (unsigned long long)(kU32Max - i);
+}
+
+__global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_request *req, uint32_t n_max,
+                                                            const uint32_t *n_ptr, unsigned long long *first,
+                                                            uint32_t gen, uint32_t n_blocks) {
+  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+  if (i >= n_max || i >= *n_ptr) return;
+  if (request_inserts(req, i, n_blocks)) atomicMax(&first[req[i].address], first_key(gen, i));
 }
 
 // 1 when request i is the first for its address and its block can take a child level (the
-// builder refines exactly the blocks below max_depth that can have children).
-__global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_request *req, uint32_t n,
-                                                           const uint32_t *keys, const uint32_t *vals, uint32_t mask,
-                                                           const uint4 *blocks, uint32_t n_blocks,
+// builder refines exactly the blocks below max_depth that can have children); 0 beyond the
+// stored requests.
+__global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_request *req, uint32_t n_max,
+                                                           const uint32_t *n_ptr, const unsigned long long *first,
+                                                           uint32_t gen, const uint4 *blocks, uint32_t n_blocks,
                                                            uint32_t max_depth, uint32_t *flags) {
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-  if (i >= n) return;
-  const svo_child_request r = req[i];
-  // requests request_first_kernel never inserted are not looked up, nor requests right after
-  // an inserted request for the same address
-  if (r.address >= n_blocks || r.tree_size == 0 ||
-      (i > 0 && req[i - 1].address == r.address && req[i - 1].tree_size != 0)) {
-    flags[i] = 0u;
-    return;
-  }
-  uint32_t h = addr_hash(r.address, mask), first = kU32Max;
-  for (uint32_t probe = 0; probe <= mask; probe++) {
-    const uint32_t k = keys[h];
-    if (k == r.address) {
-      first = vals[h];
-      break;
-    }
-    if (k == kU32Max) break;  // an empty slot ends the probe sequence
-    h = (h + 1u) & mask;
-  }
-  bool ok = first == i && r.address < n_blocks && (uint32_t)r.depth + 1u <= max_depth && r.depth >= 1;
+  if (i >= n_max) return;
+  bool ok = i < *n_ptr && request_inserts(req, i, n_blocks);
   if (ok) {
-    const uint4 blk = blocks[r.address];
-    // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
-    const uint32_t chunk = blk.y & 0xffffu;
-    ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
+    const svo_child_request r = req[i];
+    ok = first[r.address] == first_key(gen, i) && (uint32_t)r.depth + 1u <= max_depth && r.depth >= 1;
+    if (ok) {
+      const uint4 blk = blocks[r.address];
+      // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
+      const uint32_t chunk = blk.y & 0xffffu;
+      ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
+    }
   }
   flags[i] = ok ? 1u : 0u;
 }
 
 // Request list[k] becomes graft record k and brick blocks [8k, 8k+8); eight threads a brick.
 __global__ __launch_bounds__(kWg) void brick_kernel(TerrainView t, const svo_child_request *req, const uint32_t *list,
-                                                    uint32_t nb, svo_grafting *records, uint4 *bricks) {
+                                                    uint32_t nb_max, const uint32_t *unique, svo_grafting *records,
+                                                    uint4 *bricks) {
   const uint32_t tid = blockIdx.x * kWg + threadIdx.x;
   const uint32_t k = tid >> 3, c = tid & 7u;
-  if (k >= nb) return;
+  if (k >= nb_max || k >= *unique) return;
   const svo_child_request r = req[list[k]];
   const uint32_t D = r.depth, s = 64u - D;
   const uint32_t x = (uint32_t)(r.location[0] >> s), y = (uint32_t)(r.location[1] >> s),
@@ -671,23 +664,28 @@ int launch_seed(const TerrainView &t, uint16_t *bases, uint32_t B, uint4 *blocks
   return rc_of();
 }
 
-int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t mask,
-                          const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
+int launch_request_count(const uint32_t *request_id, uint32_t stored, uint32_t *counters, hipStream_t s) {
+  hipLaunchKernelGGL(request_count_kernel, dim3(1), dim3(64), 0, s, request_id, stored, counters);
+  return rc_of();
+}
+
+int launch_request_unique(const svo_child_request *req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
+                          uint32_t gen, const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s) {
-  if (!n) return 0;
-  hipLaunchKernelGGL(request_first_kernel, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, s, req, n, keys, vals, mask,
+  if (!n_max) return 0;
+  hipLaunchKernelGGL(request_first_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, n, first, gen,
                      n_blocks);
   if (rc_of()) return -1;
-  hipLaunchKernelGGL(request_flag_kernel, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, s, req, n, keys, vals, mask,
+  hipLaunchKernelGGL(request_flag_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, n, first, gen,
                      blocks, n_blocks, max_depth, flags);
   return rc_of();
 }
 
-int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb,
-                  svo_grafting *records, uint4 *bricks, hipStream_t s) {
-  if (!nb) return 0;
-  hipLaunchKernelGGL(brick_kernel, dim3((uint32_t)((8ull * nb + kWg - 1) / kWg)), dim3(kWg), 0, s, t, req, list, nb,
-                     records, bricks);
+int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb_max,
+                  const uint32_t *unique, svo_grafting *records, uint4 *bricks, hipStream_t s) {
+  if (!nb_max) return 0;
+  hipLaunchKernelGGL(brick_kernel, dim3((uint32_t)((8ull * nb_max + kWg - 1) / kWg)), dim3(kWg), 0, s, t, req, list,
+                     nb_max, unique, records, bricks);
   return rc_of();
 }
 

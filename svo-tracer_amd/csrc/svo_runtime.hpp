@@ -63,6 +63,13 @@ int enqueue_tree(Context *ctx, int kernel, uint32_t n);
 // Graft of device-resident records and graft blocks against the context's pool.
 int graft_device(Context *ctx, const svo_update_input_data &in, const svo_grafting *records, uint32_t n,
                  const svo_block *graft_blocks, uint32_t n_graft_blocks, svo_tree_stats *stats);
+// The same with the record count on the device (at most n_max records): nothing is read
+// back; *counters gets the device address of the dispatch's counter block (kTcSlots u32),
+// whose statistics stats_from() reports once the caller has read it.
+int graft_device_counted(Context *ctx, const svo_update_input_data &in, const svo_grafting *records, uint32_t n_max,
+                         const uint32_t *n_dev, const svo_block *graft_blocks, uint32_t n_graft_blocks,
+                         const uint32_t **counters);
+int stats_from(const uint32_t *counters, svo_tree_stats *out);
 // Garbage collection's reclaim of stale live groups (age > in.offset) in the context's pool.
 int reclaim_device(Context *ctx, const svo_update_input_data &in, uint32_t *reclaimed);
 int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb);
@@ -127,6 +134,8 @@ struct Context {
   // scratch of evaluateMaterial: per-entry counts and first ids, chunk sums, queue flags
   std::shared_ptr<DeviceBuffer> material_scratch[5];
   uint32_t *stream_host = nullptr;
+  // generation of the streaming update's first-request table (0: never used)
+  uint32_t stream_gen = 0;
   // span sums of the multi-work-group scans (tree management, streaming, device builds)
   std::shared_ptr<DeviceBuffer> scan_sums;
   uint32_t *id_host = nullptr;  // pinned word for the queue-counter read-backs (svo_reset_id)

@@ -32,12 +32,17 @@ int launch_seed(const TerrainView &t, uint16_t *bases, uint32_t B, uint4 *blocks
 // its block address, and the block can take a child level (address in range, childless,
 // 1 <= depth < max_depth, an octant with both air and geometry).  keys / vals are a
 // table of mask+1 entries, filled with UINT32_MAX beforehand.
-int launch_request_unique(const svo_child_request *req, uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t mask,
-                          const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
+// counters[0] = min(*request_id, stored), counters[1] = *request_id
+int launch_request_count(const uint32_t *request_id, uint32_t stored, uint32_t *counters, hipStream_t s);
+// flags[i] (i < n_max) = 1 for the first request of each block that can take a child level
+// among the *n stored requests; first: u64 per block of the pool, entries of generations
+// below gen are ignored (zero-filled when allocated)
+int launch_request_unique(const svo_child_request *req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
+                          uint32_t gen, const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s);
-// Request list[k] (k < nb) becomes graft record k and brick blocks [8k, 8k + 8).
-int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb,
-                  svo_grafting *records, uint4 *bricks, hipStream_t s);
+// bricks of the first min(*unique, nb_max) listed requests
+int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb_max,
+                  const uint32_t *unique, svo_grafting *records, uint4 *bricks, hipStream_t s);
 // rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.  With
 // a list, the flagged entries of rank below limit are listed in rank order.  span_sums:
 // scratch of the scan over the counts (scan_span_sums_bytes of the count of chunks).

@@ -49,8 +49,23 @@ enum : uint32_t {
   kTcCollected = 12,
   kTcPlaced = 13,
   kTcUnlinks = 14,
+  kTcGot = 16,        // collection: groups collected so far
+  kTcNeed = 17,       // graft: groups the grafts need (scan total of need)
+  kTcK = 18,          // collection: groups this pass claims
+  kTcFrom = 19,       // collection: where this pass's groups start in the list
   kTcSlots = 64
 };
+// kTcOverflow bits
+enum : uint32_t { kOverflowFront = 1, kOverflowEvents = 2 };
+
+// A count a launch needs: the value itself, or (p set) a device location holding it, with v
+// then its host-known upper bound (grids are sized by v; kernels read min(*p, v)).
+struct DevCount {
+  uint32_t v;
+  const uint32_t *p;
+  __device__ __forceinline__ uint32_t get() const { return p ? (*p < v ? *p : v) : v; }
+};
+inline DevCount host_count(uint32_t v) { return DevCount{v, nullptr}; }
 
 // Scratch of a management dispatch (device pointers; sizes in elements).
 struct TreeScratch {
@@ -92,8 +107,11 @@ int launch_prune_winners(const PoolView &p, const svo_pruning *rec, uint32_t n, 
                          hipStream_t s);
 int launch_prune_roots(const PoolView &p, const svo_pruning *rec, uint32_t n, const TreeScratch &t,
                        hipStream_t s);
-// one level of the free walk: reads front[in], writes front[1-in]
-int launch_free_level(const PoolView &p, const TreeScratch &t, uint32_t in, bool use_stop, hipStream_t s);
+// The whole free walk from the groups queued in front[0], level by level, in one cooperative
+// launch (grid-wide barriers between levels); leaves both frontier counts at zero.
+int launch_free_walk(const PoolView &p, const TreeScratch &t, bool use_stop, hipStream_t s);
+// Multi-parent unlinks queued by the walk, in order; clears the event count and flags more
+// events than the table holds in counters[kTcOverflow].
 int launch_unlink(const PoolView &p, const TreeScratch &t, hipStream_t s);
 int launch_prune_apply(const PoolView &p, const svo_pruning *rec, uint32_t n, const svo_block *surface,
                        const TreeScratch &t, hipStream_t s);
@@ -102,14 +120,19 @@ int launch_prune_apply(const PoolView &p, const svo_pruning *rec, uint32_t n, co
 // live_only (reclaim) only stale live groups are candidates; mark gives claimed groups `tick`.
 int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only,
                          const TreeScratch &t, hipStream_t s);
-int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only, uint32_t from,
-                           uint32_t limit, const TreeScratch &t, hipStream_t s);
-int launch_collect_claim(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s);
-int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, uint32_t tick, const TreeScratch &t,
-                        hipStream_t s);
+// this pass takes k = min(candidates, need - collected) groups from list position collected
+// (counters kTcK, kTcFrom); advance adds k to the collected count
+int launch_collect_limit(const TreeScratch &t, DevCount need, hipStream_t s);
+int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only,
+                           const TreeScratch &t, hipStream_t s);
+// k_max: a bound on this pass's k (the list capacity)
+int launch_collect_claim(const PoolView &p, uint32_t k_max, const TreeScratch &t, hipStream_t s);
+int launch_collect_mark(const PoolView &p, uint32_t k_max, uint32_t tick, const TreeScratch &t, hipStream_t s);
+int launch_collect_advance(const TreeScratch &t, hipStream_t s);
 
 // graft
-int launch_graft_sizes(const svo_grafting *rec, uint32_t n, uint32_t n_gb, const TreeScratch &t, hipStream_t s);
+// n.v records are processed; those at or beyond n.get() count as absent (size and need 0)
+int launch_graft_sizes(const svo_grafting *rec, DevCount n, uint32_t n_gb, const TreeScratch &t, hipStream_t s);
 int launch_graft_structure(const svo_grafting *rec, uint32_t n, const svo_block *gb, uint32_t n_gb,
                            const TreeScratch &t, hipStream_t s);
 // Exclusive scan of n u32; the sum goes to *total.  Above 2 x 16384 elements the scan needs
@@ -117,11 +140,12 @@ int launch_graft_structure(const svo_grafting *rec, uint32_t n, const svo_block 
 size_t scan_span_sums_bytes(uint32_t n);
 int launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total, uint32_t *span_sums,
                     hipStream_t s);
-int launch_graft_winners(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
-int launch_graft_roots(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
+// got: the groups the collection gathered
+int launch_graft_winners(const PoolView &p, uint32_t n, DevCount got, const TreeScratch &t, hipStream_t s);
+int launch_graft_roots(const PoolView &p, DevCount n, DevCount got, const TreeScratch &t, hipStream_t s);
 // total_need: the sum of need over the n records (the scan total of need into start).
-int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, uint32_t total_need,
+int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, DevCount total_need,
                        const svo_block *gb, const TreeScratch &t, hipStream_t s);
-int launch_graft_release(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s);
+int launch_graft_release(const PoolView &p, uint32_t n, DevCount got, const TreeScratch &t, hipStream_t s);
 
 }  // namespace svo

@@ -1,9 +1,10 @@
 // Host side of tree management: svo_prune, svo_graft, svo_collect and the Prune / Graft
 // entries of svo_enqueue (KernelName.Prune / KernelName.Graft, ComputeManager.cs:313-318).
 //
-// Every phase runs on the device; the host only sequences the kernels and reads a few
-// counters back (the size of each level of the free walk, the candidates of each
-// collection pass), so a dispatch costs a handful of small synchronisations.
+// Every phase runs on the device and sizes itself from device-side counts (the free walk is
+// one cooperative launch over all levels); the host sequences the kernels and reads the
+// counters back once at the end of a dispatch, plus once after each collection pass that may
+// be followed by another.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,6 +57,28 @@ uint32_t pow2_at_least(uint64_t v) {
   while (p < v) p <<= 1;
   return (uint32_t)std::min<uint64_t>(p, 1ull << 31);
 }
+
+}  // namespace
+
+// The statistics of a dispatch from its counter block (kTcSlots words read back), after the
+// overflow flags the device may have raised.
+int stats_from(const uint32_t *c, svo_tree_stats *out) {
+  if (c[kTcOverflow] & kOverflowFront) return fail(SVO_ERR_CAPACITY, "tree: free walk frontier overflow");
+  if (c[kTcOverflow] & kOverflowEvents)
+    return fail(SVO_ERR_CAPACITY, "tree: more than 4096 multi-parent unlinks in one dispatch");
+  if (!out) return SVO_OK;
+  std::memset(out, 0, sizeof *out);
+  out->applied = c[kTcApplied];
+  out->skipped = c[kTcSkipped];
+  out->invalid = c[kTcInvalid];
+  out->requests = c[kTcRequests];
+  out->collected = c[kTcGot];
+  out->placed = c[kTcPlaced];
+  out->unlinks = c[kTcUnlinks];
+  return SVO_OK;
+}
+
+namespace {
 
 struct Op {
   Context *ctx;
@@ -147,55 +170,41 @@ struct Op {
   int launch(int r, const char *what) { return r ? fail(SVO_ERR_HIP, std::string("tree: launch ") + what) : SVO_OK; }
 
   // Frees the subtrees queued in front[0] (count in its counter), level by level, then
-  // applies the multi-parent unlink events in order.
+  // applies the multi-parent unlink events in order.  Nothing is read back: an overflow of
+  // the frontier or of the event table is flagged on the device and reported by finish().
   int free_walk(bool use_stop) {
-    uint32_t in = 0;
-    int rc;
-    for (uint32_t level = 0; level < kMaxFreeLevels; level++) {
-      if ((rc = read(kTcFront0 + in, 1))) return rc;
-      if (host[kTcFront0 + in] == 0) break;
-      if ((rc = launch(tree_fill_u32(ts.counters + kTcFront0 + 1 - in, 0, 1, ctx->stream), "fill"))) return rc;
-      if ((rc = launch(launch_free_level(pv, ts, in, use_stop, ctx->stream), "free level"))) return rc;
-      in = 1 - in;
-    }
-    if ((rc = read(kTcEvents, 2))) return rc;
-    if (host[kTcOverflow]) return fail(SVO_ERR_CAPACITY, "tree: free walk frontier overflow");
-    if (host[kTcEvents] > ts.event_capacity)
-      return fail(SVO_ERR_CAPACITY, "tree: more than 4096 multi-parent unlinks in one dispatch");
-    if (host[kTcEvents] && (rc = launch(launch_unlink(pv, ts, ctx->stream), "unlink"))) return rc;
-    // both frontier counters and the event counter start the next walk at zero
-    return launch(tree_fill_u32(ts.counters + kTcFront0, 0, 3, ctx->stream), "fill");
+    int rc = launch(launch_free_walk(pv, ts, use_stop, ctx->stream), "free walk");
+    if (rc) return rc;
+    return launch(launch_unlink(pv, ts, ctx->stream), "unlink");
   }
 
-  // Collection (graft, wavefront.cl:1277-1305): up to `need` groups into ts.list.
-  int collect(const svo_update_input_data &in, uint32_t need, uint32_t *got) {
-    *got = 0;
+  // Collection (graft, wavefront.cl:1277-1305): up to `need` groups (at most need_max) into
+  // ts.list; the count collected accumulates in counters[kTcGot].  One pass per age
+  // threshold (offset, offset/2, ..., 0); after each pass that may be followed by another the
+  // host reads whether the need is met.
+  int collect(const svo_update_input_data &in, DevCount need, uint32_t need_max) {
     void *p;
-    int rc = grow(kTbList, (size_t)std::max(need, 1u) * 4, &p);
+    int rc = grow(kTbList, (size_t)std::max(need_max, 1u) * 4, &p);
     if (rc) return rc;
     ts.list = (uint32_t *)p;
-    ts.list_capacity = need;
-    if (!need) return SVO_OK;
+    ts.list_capacity = need_max;
+    if (!need_max) return SVO_OK;
     if ((rc = launch(tree_fill_u32(ts.claimed, 0, ts.claimed_words + 1, ctx->stream), "fill"))) return rc;
     uint32_t ng = pv.n_usages;
     if (in.memory_size) ng = (uint32_t)std::min<uint64_t>(((uint64_t)in.memory_size + 7u) / 8u, ng);
-    uint32_t count = 0, offset = in.offset;
-    while (count < need) {
+    for (uint32_t offset = in.offset;; offset >>= 1) {
       if ((rc = launch(launch_collect_count(pv, ng, offset, false, ts, ctx->stream), "collect count"))) return rc;
-      if ((rc = read(kTcTotal, 1))) return rc;
-      const uint32_t k = std::min(host[kTcTotal], need - count);
-      if (k) {
-        if ((rc = launch(launch_collect_scatter(pv, ng, offset, false, count, k, ts, ctx->stream), "collect scatter")))
-          return rc;
-        if ((rc = launch(launch_collect_claim(pv, count, k, ts, ctx->stream), "collect claim"))) return rc;
-        if ((rc = free_walk(true))) return rc;
-        if ((rc = launch(launch_collect_mark(pv, count, k, pv.now, ts, ctx->stream), "collect mark"))) return rc;
-        count += k;
-      }
+      if ((rc = launch(launch_collect_limit(ts, need, ctx->stream), "collect limit"))) return rc;
+      if ((rc = launch(launch_collect_scatter(pv, ng, offset, false, ts, ctx->stream), "collect scatter"))) return rc;
+      if ((rc = launch(launch_collect_claim(pv, need_max, ts, ctx->stream), "collect claim"))) return rc;
+      if ((rc = free_walk(true))) return rc;
+      if ((rc = launch(launch_collect_mark(pv, need_max, pv.now, ts, ctx->stream), "collect mark"))) return rc;
+      if ((rc = launch(launch_collect_advance(ts, ctx->stream), "collect advance"))) return rc;
       if (offset == 0) break;
-      offset >>= 1;
+      if ((rc = read(kTcGot, 2))) return rc;  // collected so far and (device need) the need
+      const uint32_t nd = need.p ? std::min(host[kTcNeed], need.v) : need.v;
+      if (host[kTcGot] >= nd) break;
     }
-    *got = count;
     return SVO_OK;
   }
 
@@ -210,33 +219,26 @@ struct Op {
     if ((rc = launch(launch_collect_count(pv, ng, in.offset, true, ts, ctx->stream), "reclaim count"))) return rc;
     if ((rc = read(kTcTotal, 1))) return rc;
     const uint32_t k = host[kTcTotal];
-    if (!k) return SVO_OK;
+    if (!k) return finish(nullptr);
     void *p;
     if ((rc = grow(kTbList, (size_t)k * 4, &p))) return rc;
     ts.list = (uint32_t *)p;
     ts.list_capacity = k;
-    if ((rc = launch(launch_collect_scatter(pv, ng, in.offset, true, 0, k, ts, ctx->stream), "reclaim scatter")))
-      return rc;
-    if ((rc = launch(launch_collect_claim(pv, 0, k, ts, ctx->stream), "reclaim claim"))) return rc;
+    if ((rc = launch(launch_collect_limit(ts, host_count(k), ctx->stream), "reclaim limit"))) return rc;
+    if ((rc = launch(launch_collect_scatter(pv, ng, in.offset, true, ts, ctx->stream), "reclaim scatter"))) return rc;
+    if ((rc = launch(launch_collect_claim(pv, k, ts, ctx->stream), "reclaim claim"))) return rc;
     if ((rc = free_walk(true))) return rc;
-    if ((rc = launch(launch_collect_mark(pv, 0, k, 0u, ts, ctx->stream), "reclaim mark"))) return rc;
+    if ((rc = launch(launch_collect_mark(pv, k, 0u, ts, ctx->stream), "reclaim mark"))) return rc;
+    if ((rc = finish(nullptr))) return rc;
     *got = k;
     return SVO_OK;
   }
 
-  int stats(svo_tree_stats *out, uint32_t collected) {
-    if (!out) return SVO_OK;
-    int rc = read(kTcApplied, 7);
+  // Reads the counters once, reports a flagged overflow and fills `out` (may be null).
+  int finish(svo_tree_stats *out) {
+    int rc = read(0, kTcSlots);
     if (rc) return rc;
-    std::memset(out, 0, sizeof *out);
-    out->applied = host[kTcApplied];
-    out->skipped = host[kTcSkipped];
-    out->invalid = host[kTcInvalid];
-    out->requests = host[kTcRequests];
-    out->collected = collected;
-    out->placed = host[kTcPlaced];
-    out->unlinks = host[kTcUnlinks];
-    return SVO_OK;
+    return stats_from(host, out);
   }
 };
 
@@ -262,11 +264,15 @@ int prune_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
   if ((rc = op.launch(launch_prune_roots(pv, rec, n, ts, s), "prune roots"))) return rc;
   if ((rc = op.free_walk(false))) return rc;
   if ((rc = op.launch(launch_prune_apply(pv, rec, n, surface, ts, s), "prune apply"))) return rc;
-  return op.stats(stats, 0);
+  return op.finish(stats);
 }
 
+// dev_records: when set, the number of records lives there on the device (at most n) and the
+// dispatch reads nothing back: its counters stay in the op's counter block for the caller
+// (graft_counters) to read with its own.
 int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, const svo_grafting *rec, uint32_t n,
-               const svo_block *gb, uint32_t n_gb, const uint64_t *locs, uint32_t n_locs, svo_tree_stats *stats) {
+               const svo_block *gb, uint32_t n_gb, const uint64_t *locs, uint32_t n_locs, svo_tree_stats *stats,
+               const uint32_t *dev_records = nullptr) {
   Op op{ctx};
   int rc = op.init(pool, in);
   if (rc) return rc;
@@ -283,25 +289,31 @@ int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
   op.ts.need = (uint32_t *)p;
   if ((rc = op.grow(kTbStart, per_rec, &p))) return rc;
   op.ts.start = (uint32_t *)p;
+  const DevCount records{n, dev_records};
   // structure: sizes, where each graft's numbering lives, pre-order numbering, needs
-  if ((rc = op.launch(launch_graft_sizes(rec, n, n_gb, op.ts, s), "graft sizes"))) return rc;
+  if ((rc = op.launch(launch_graft_sizes(rec, records, n_gb, op.ts, s), "graft sizes"))) return rc;
   uint32_t *sums = nullptr;
   if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(n), &sums))) return rc;
   if ((rc = op.launch(launch_scan_u32(op.ts.size, op.ts.where, n, op.ts.counters + kTcTotal, sums, s), "scan")))
     return rc;
-  if ((rc = op.read(kTcTotal, 1))) return rc;
-  const size_t numbering = (size_t)op.host[kTcTotal] + 1;
-  if ((rc = op.grow(kTbNumber, numbering * 4, &p))) return rc;
+  // groups of all grafts: read back for host records; for device records every graft's
+  // groups lie in its own part of the graft blocks, so n_gb / 8 bounds them
+  uint32_t groups = n_gb / 8u;
+  if (!dev_records) {
+    if ((rc = op.read(kTcTotal, 1))) return rc;
+    groups = op.host[kTcTotal];
+  }
+  if ((rc = op.grow(kTbNumber, ((size_t)groups + 1) * 4, &p))) return rc;
   op.ts.number = (uint32_t *)p;
-  if ((rc = op.grow(kTbParent, numbering * 4, &p))) return rc;
+  if ((rc = op.grow(kTbParent, ((size_t)groups + 1) * 4, &p))) return rc;
   op.ts.parent = (uint32_t *)p;
   if ((rc = op.launch(launch_graft_structure(rec, n, gb, n_gb, op.ts, s), "graft structure"))) return rc;
-  if ((rc = op.launch(launch_scan_u32(op.ts.need, op.ts.start, n, op.ts.counters + kTcTotal, sums, s), "scan")))
+  if ((rc = op.launch(launch_scan_u32(op.ts.need, op.ts.start, n, op.ts.counters + kTcNeed, sums, s), "scan")))
     return rc;
-  if ((rc = op.read(kTcTotal, 1))) return rc;
-  const uint32_t total_need = op.host[kTcTotal];
-  uint32_t got = 0;
-  if ((rc = op.collect(in, total_need, &got))) return rc;
+  // every need is at most its graft's size, so `groups` bounds the total need
+  const DevCount need{groups, op.ts.counters + kTcNeed};
+  if ((rc = op.collect(in, need, groups))) return rc;
+  const DevCount got{groups, op.ts.counters + kTcGot};
   const PoolView &pv = op.pv;
   const TreeScratch &ts = op.ts;
   const uint8_t *rb = reinterpret_cast<const uint8_t *>(rec);
@@ -312,11 +324,12 @@ int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
                       "graft requests")))
     return rc;
   if ((rc = op.launch(launch_graft_winners(pv, n, got, ts, s), "graft winners"))) return rc;
-  if ((rc = op.launch(launch_graft_roots(pv, n, got, ts, s), "graft roots"))) return rc;
+  if ((rc = op.launch(launch_graft_roots(pv, records, got, ts, s), "graft roots"))) return rc;
   if ((rc = op.free_walk(true))) return rc;
-  if ((rc = op.launch(launch_graft_place(pv, rec, n, total_need, gb, ts, s), "graft place"))) return rc;
+  if ((rc = op.launch(launch_graft_place(pv, rec, n, need, gb, ts, s), "graft place"))) return rc;
   if ((rc = op.launch(launch_graft_release(pv, n, got, ts, s), "graft release"))) return rc;
-  return op.stats(stats, got);
+  if (dev_records) return SVO_OK;
+  return op.finish(stats);
 }
 
 Pool pool_from_context(Context *ctx) {
@@ -381,6 +394,16 @@ struct Bound {
 int graft_device(Context *ctx, const svo_update_input_data &in, const svo_grafting *records, uint32_t n,
                  const svo_block *graft_blocks, uint32_t n_graft_blocks, svo_tree_stats *stats) {
   return graft_impl(ctx, pool_from_context(ctx), in, records, n, graft_blocks, n_graft_blocks, nullptr, 0, stats);
+}
+
+int graft_device_counted(Context *ctx, const svo_update_input_data &in, const svo_grafting *records, uint32_t n_max,
+                         const uint32_t *n_dev, const svo_block *graft_blocks, uint32_t n_graft_blocks,
+                         const uint32_t **counters) {
+  int rc = graft_impl(ctx, pool_from_context(ctx), in, records, n_max, graft_blocks, n_graft_blocks, nullptr, 0,
+                      nullptr, n_dev);
+  if (rc) return rc;
+  *counters = (const uint32_t *)ctx->tree_scratch[kTbCounters]->ptr;
+  return SVO_OK;
 }
 
 int reclaim_device(Context *ctx, const svo_update_input_data &in, uint32_t *reclaimed) {
@@ -467,8 +490,9 @@ extern "C" int svo_collect(svo_context *ctx, const svo_update_input_data *input,
   Op op{ctx};
   int rc = op.init(pool_from_context(ctx), *input);
   if (rc) return rc;
-  uint32_t got = 0;
-  if ((rc = op.collect(*input, need, &got))) return rc;
+  if ((rc = op.collect(*input, host_count(need), need))) return rc;
+  if ((rc = op.finish(nullptr))) return rc;
+  const uint32_t got = op.host[kTcGot];
   if (groups && got) {
     hipError_t e = hipMemcpyAsync(groups, op.ts.list, (size_t)got * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

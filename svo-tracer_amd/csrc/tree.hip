@@ -8,6 +8,8 @@
 // holding the largest record index).  Collection ranks candidates in ascending group
 // order with an order-preserving count / scan / scatter, one pass per age threshold.
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "svo_device.hpp"
 #include "svo_tree_kernels.hpp"
 
@@ -450,11 +452,10 @@ __global__ __launch_bounds__(kFlat) void prune_fields_kernel(PoolView p, const s
 // the free walk (dereference / helpDereference, wavefront.cl:1097-1189)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kFlat) void free_level_kernel(PoolView p, const uint2 *in, const uint32_t *in_count,
-                                                           uint2 *out, uint32_t *out_count, uint32_t cap,
-                                                           uint32_t *overflow, uint2 *events, uint32_t *ev_count,
-                                                           uint32_t ev_cap, const uint32_t *stop) {
-  const uint32_t n = *in_count < cap ? *in_count : cap;
+// One level: frees the n groups of `in` and queues their children into `out`.
+__device__ void free_level(const PoolView &p, const uint2 *in, uint32_t n, uint2 *out, uint32_t *out_count,
+                           uint32_t cap, uint32_t *overflow, uint2 *events, uint32_t *ev_count, uint32_t ev_cap,
+                           const uint32_t *stop) {
   for (uint32_t i = blockIdx.x * kFlat + threadIdx.x; i < n; i += gridDim.x * kFlat) {
     const uint2 e = in[i];
     const uint32_t g = e.x;
@@ -474,6 +475,28 @@ __global__ __launch_bounds__(kFlat) void free_level_kernel(PoolView p, const uin
       const uint32_t c = p.blocks[b].child;
       if (c != kU32Max) push(out, out_count, cap, overflow, c >> 3, b);
     }
+  }
+}
+
+// Every level in one launch (cooperative: all work-groups are resident).  The frontier size
+// is read after a grid-wide barrier, so every work-group sees the same value and leaves the
+// loop at the same level; the consumed frontier's count is cleared between two barriers,
+// before the next level appends to it.
+__global__ __launch_bounds__(kFlat) void free_walk_kernel(PoolView p, uint2 *f0, uint2 *f1, uint32_t *counters,
+                                                          uint32_t cap, uint2 *events, uint32_t ev_cap,
+                                                          const uint32_t *stop) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  uint32_t in = 0;
+  for (uint32_t level = 0; level < kMaxFreeLevels; level++) {
+    const uint32_t c = __hip_atomic_load(&counters[kTcFront0 + in], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t n = c < cap ? c : cap;
+    if (n == 0) break;
+    free_level(p, in ? f1 : f0, n, in ? f0 : f1, &counters[kTcFront0 + 1u - in], cap, &counters[kTcOverflow], events,
+               &counters[kTcEvents], ev_cap, stop);
+    grid.sync();
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kTcFront0 + in] = 0;
+    grid.sync();
+    in ^= 1u;
   }
 }
 
@@ -540,6 +563,11 @@ __global__ __launch_bounds__(kScanWg) void unlink_kernel(PoolView p, const uint2
       done++;
     }
     counters[kTcUnlinks] += done;
+    // the next walk starts with no events and empty frontiers
+    if (counters[kTcEvents] > cap) counters[kTcOverflow] |= kOverflowEvents;
+    counters[kTcEvents] = 0;
+    counters[kTcFront0] = 0;
+    counters[kTcFront1] = 0;
   }
 }
 
@@ -572,11 +600,25 @@ __global__ __launch_bounds__(kFlat) void collect_count_kernel(PoolView p, uint32
   if (threadIdx.x == 0) chunk_counts[blockIdx.x] = all;
 }
 
+// k = min(candidates of this pass, need - collected), taken from list position `collected`
+__global__ void collect_limit_kernel(uint32_t *counters, DevCount need) {
+  if (threadIdx.x != 0) return;
+  const uint32_t got = counters[kTcGot], nd = need.get();
+  const uint32_t want = nd > got ? nd - got : 0u, total = counters[kTcTotal];
+  counters[kTcK] = total < want ? total : want;
+  counters[kTcFrom] = got;
+}
+
+__global__ void collect_advance_kernel(uint32_t *counters) {
+  if (threadIdx.x == 0) counters[kTcGot] += counters[kTcK];
+}
+
 __global__ __launch_bounds__(kFlat) void collect_scatter_kernel(PoolView p, uint32_t ng, uint32_t offset,
                                                                 uint32_t live_only, const uint32_t *claimed,
-                                                                const uint32_t *chunk_offsets, uint32_t from,
-                                                                uint32_t limit, uint32_t *list) {
+                                                                const uint32_t *chunk_offsets,
+                                                                const uint32_t *counters, uint32_t *list) {
   __shared__ uint32_t s_waves[kFlat / 64];
+  const uint32_t from = counters[kTcFrom], limit = counters[kTcK];
   const uint32_t g0 = blockIdx.x * kCollectChunk + threadIdx.x * kPerThread;
   bool is[kPerThread];
   uint32_t c = 0;
@@ -595,12 +637,13 @@ __global__ __launch_bounds__(kFlat) void collect_scatter_kernel(PoolView p, uint
 
 // Claims list[from, from+k): marks them, cuts stale live groups from their parents and
 // queues what hangs below them for freeing.
-__global__ __launch_bounds__(kFlat) void collect_claim_kernel(PoolView p, const uint32_t *list, uint32_t from,
-                                                              uint32_t k, uint32_t *claimed, uint2 *front,
-                                                              uint32_t *counters, uint32_t cap) {
+__global__ __launch_bounds__(kFlat) void collect_claim_kernel(PoolView p, const uint32_t *list, uint32_t k_max,
+                                                              uint32_t *claimed, uint2 *front, uint32_t *counters,
+                                                              uint32_t cap) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
+  const uint32_t k = counters[kTcK] < k_max ? counters[kTcK] : k_max;
   if (i >= k) return;
-  const uint32_t g = list[from + i];
+  const uint32_t g = list[counters[kTcFrom] + i];
   atomicOr(&claimed[g >> 5], 1u << (g & 31u));
   if (p.usages[g].tick == 0) return;
   const uint32_t pb = p.usages[g].parent;
@@ -614,11 +657,12 @@ __global__ __launch_bounds__(kFlat) void collect_claim_kernel(PoolView p, const 
 }
 
 // Claimed groups get `tick`: the frame tick when collected for a graft, 0 when reclaimed.
-__global__ __launch_bounds__(kFlat) void collect_mark_kernel(PoolView p, const uint32_t *list, uint32_t from,
-                                                             uint32_t k, uint32_t tick) {
+__global__ __launch_bounds__(kFlat) void collect_mark_kernel(PoolView p, const uint32_t *list, uint32_t k_max,
+                                                             const uint32_t *counters, uint32_t tick) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
+  const uint32_t k = counters[kTcK] < k_max ? counters[kTcK] : k_max;
   if (i >= k) return;
-  const uint32_t g = list[from + i];
+  const uint32_t g = list[counters[kTcFrom] + i];
   p.usages[g].tick = (uint16_t)tick;
   p.usages[g].parent = kU32Max;
 }
@@ -627,10 +671,14 @@ __global__ __launch_bounds__(kFlat) void collect_mark_kernel(PoolView p, const u
 // graft (wavefront.cl:1306-1350, as intended)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kFlat) void graft_size_kernel(const svo_grafting *rec, uint32_t n, uint32_t n_gb,
+__global__ __launch_bounds__(kFlat) void graft_size_kernel(const svo_grafting *rec, DevCount n, uint32_t n_gb,
                                                            uint32_t *size) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n.v) return;
+  if (i >= n.get()) {
+    size[i] = 0u;  // beyond the records present: no groups, no need
+    return;
+  }
   const uint32_t gda = rec[i].graft_data_address, tot = rec[i].graft_total_size;
   const bool ok = tot >= 8u && (tot & 7u) == 0 && (gda & 7u) == 0 && (unsigned long long)gda + tot <= n_gb;
   size[i] = ok ? tot / 8u : 0u;
@@ -686,22 +734,28 @@ __global__ __launch_bounds__(64) void graft_structure_kernel(const svo_grafting 
 }
 
 __global__ __launch_bounds__(kFlat) void graft_winners_kernel(const ResolveOut *res, const uint32_t *start,
-                                                              const uint32_t *need, uint32_t n, uint32_t got,
+                                                              const uint32_t *need, uint32_t n, DevCount got_count,
                                                               unsigned long long *keys, uint32_t *vals,
                                                               uint32_t mask) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
   if (i >= n) return;
+  const uint32_t got = got_count.get();
   if (res[i].status == kResOk && start[i] + need[i] <= got) win_insert(keys, vals, mask, win_key(kClsGraft, res[i].addr), i);
 }
 
 // Decides which grafts are linked and queues what their targets held before.
 __global__ __launch_bounds__(kFlat) void graft_roots_kernel(PoolView p, ResolveOut *res, const uint32_t *start,
-                                                            const uint32_t *need, uint32_t n, uint32_t got,
+                                                            const uint32_t *need, DevCount n, DevCount got_count,
                                                             const unsigned long long *keys, const uint32_t *vals,
                                                             uint32_t mask, uint2 *front, uint32_t *counters,
                                                             uint32_t cap) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n.v) return;
+  if (i >= n.get()) {  // beyond the records present: not placed, not counted
+    res[i].placed = 0u;
+    return;
+  }
+  const uint32_t got = got_count.get();
   const ResolveOut o = res[i];
   const bool placed = o.status == kResOk && start[i] + need[i] <= got &&
                       win_get(keys, vals, mask, win_key(kClsGraft, o.addr)) == i;
@@ -722,10 +776,10 @@ __global__ __launch_bounds__(kFlat) void graft_place_kernel(PoolView p, const sv
                                                             const ResolveOut *res, const uint32_t *start,
                                                             const uint32_t *where, const uint32_t *number,
                                                             const uint32_t *parent, const uint32_t *list, uint32_t n,
-                                                            uint32_t total_need) {
+                                                            DevCount need_count) {
   const uint32_t t = blockIdx.x * kFlat + threadIdx.x;
   const uint32_t g = t >> 3;
-  if (g >= total_need) return;
+  if (g >= need_count.v || g >= need_count.get()) return;
   uint32_t lo = 0, hi = n;  // start[lo] <= g < start[hi] (start[n] taken as total_need)
   while (hi - lo > 1u) {
     const uint32_t mid = lo + (hi - lo) / 2u;
@@ -758,9 +812,9 @@ __global__ __launch_bounds__(kFlat) void graft_link_kernel(PoolView p, const Res
 
 // Collected groups no linked graft uses go back to the free pool.
 __global__ __launch_bounds__(kFlat) void graft_release_kernel(PoolView p, const ResolveOut *res, const uint32_t *start,
-                                                              uint32_t n, const uint32_t *list, uint32_t got) {
+                                                              uint32_t n, const uint32_t *list, DevCount got) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
-  if (i >= got) return;
+  if (i >= got.v || i >= got.get()) return;
   // owner: the last record whose run starts at or before i
   uint32_t lo = 0, hi = n;
   while (hi - lo > 1u) {
@@ -868,11 +922,23 @@ int launch_prune_roots(const PoolView &p, const svo_pruning *rec, uint32_t n, co
   return launch_rc();
 }
 
-int launch_free_level(const PoolView &p, const TreeScratch &t, uint32_t in, bool use_stop, hipStream_t s) {
-  const uint32_t grid = 1024;  // grid-stride over the device-side count
-  hipLaunchKernelGGL(free_level_kernel, dim3(grid), dim3(kFlat), 0, s, p, t.front[in], &t.counters[kTcFront0 + in],
-                     t.front[1 - in], &t.counters[kTcFront0 + 1 - in], t.front_capacity, &t.counters[kTcOverflow],
-                     t.events, &t.counters[kTcEvents], t.event_capacity, use_stop ? t.claimed : nullptr);
+int launch_free_walk(const PoolView &p, const TreeScratch &t, bool use_stop, hipStream_t s) {
+  // a cooperative grid must be resident at once: at most two work-groups per CU of the
+  // current device (the walk is a grid-stride loop, so any grid size is correct)
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, free_walk_kernel, kFlat, 0) != hipSuccess)
+    return -1;
+  const uint32_t grid = (uint32_t)std::max(1, std::min(per_cu, 2) * cus);
+  PoolView pv = p;
+  uint2 *f0 = t.front[0], *f1 = t.front[1], *events = t.events;
+  uint32_t *counters = t.counters, cap = t.front_capacity, ev_cap = t.event_capacity;
+  const uint32_t *stop = use_stop ? t.claimed : nullptr;
+  void *args[] = {&pv, &f0, &f1, &counters, &cap, &events, &ev_cap, &stop};
+  if (hipLaunchCooperativeKernel(reinterpret_cast<const void *>(free_walk_kernel), dim3(grid), dim3(kFlat), args, 0,
+                                 s) != hipSuccess)
+    return -1;
   return launch_rc();
 }
 
@@ -902,26 +968,36 @@ int launch_collect_count(const PoolView &p, uint32_t n_groups, uint32_t offset, 
   return launch_scan_u32(t.chunk_counts, t.chunk_counts, chunks, &t.counters[kTcTotal], t.scan_sums, s);
 }
 
-int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only, uint32_t from,
-                           uint32_t limit, const TreeScratch &t, hipStream_t s) {
-  const uint32_t chunks = (n_groups + kCollectChunk - 1u) / kCollectChunk;
-  if (!chunks || !limit) return 0;
-  hipLaunchKernelGGL(collect_scatter_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset,
-                     live_only ? 1u : 0u, t.claimed, t.chunk_counts, from, limit, t.list);
+int launch_collect_limit(const TreeScratch &t, DevCount need, hipStream_t s) {
+  hipLaunchKernelGGL(collect_limit_kernel, dim3(1), dim3(64), 0, s, t.counters, need);
   return launch_rc();
 }
 
-int launch_collect_claim(const PoolView &p, uint32_t from, uint32_t k, const TreeScratch &t, hipStream_t s) {
-  if (!k) return 0;
-  hipLaunchKernelGGL(collect_claim_kernel, dim3(blocks_for(k)), dim3(kFlat), 0, s, p, t.list, from, k, t.claimed,
+int launch_collect_advance(const TreeScratch &t, hipStream_t s) {
+  hipLaunchKernelGGL(collect_advance_kernel, dim3(1), dim3(64), 0, s, t.counters);
+  return launch_rc();
+}
+
+int launch_collect_scatter(const PoolView &p, uint32_t n_groups, uint32_t offset, bool live_only,
+                           const TreeScratch &t, hipStream_t s) {
+  const uint32_t chunks = (n_groups + kCollectChunk - 1u) / kCollectChunk;
+  if (!chunks) return 0;
+  hipLaunchKernelGGL(collect_scatter_kernel, dim3(chunks), dim3(kFlat), 0, s, p, n_groups, offset,
+                     live_only ? 1u : 0u, t.claimed, t.chunk_counts, t.counters, t.list);
+  return launch_rc();
+}
+
+int launch_collect_claim(const PoolView &p, uint32_t k_max, const TreeScratch &t, hipStream_t s) {
+  if (!k_max) return 0;
+  hipLaunchKernelGGL(collect_claim_kernel, dim3(blocks_for(k_max)), dim3(kFlat), 0, s, p, t.list, k_max, t.claimed,
                      t.front[0], t.counters, t.front_capacity);
   return launch_rc();
 }
 
-int launch_collect_mark(const PoolView &p, uint32_t from, uint32_t k, uint32_t tick, const TreeScratch &t,
-                        hipStream_t s) {
-  if (!k) return 0;
-  hipLaunchKernelGGL(collect_mark_kernel, dim3(blocks_for(k)), dim3(kFlat), 0, s, p, t.list, from, k, tick);
+int launch_collect_mark(const PoolView &p, uint32_t k_max, uint32_t tick, const TreeScratch &t, hipStream_t s) {
+  if (!k_max) return 0;
+  hipLaunchKernelGGL(collect_mark_kernel, dim3(blocks_for(k_max)), dim3(kFlat), 0, s, p, t.list, k_max, t.counters,
+                     tick);
   return launch_rc();
 }
 
@@ -934,13 +1010,13 @@ int launch_graft_structure(const svo_grafting *rec, uint32_t n, const svo_block 
   return launch_rc();
 }
 
-int launch_graft_sizes(const svo_grafting *rec, uint32_t n, uint32_t n_gb, const TreeScratch &t, hipStream_t s) {
-  if (!n) return 0;
-  hipLaunchKernelGGL(graft_size_kernel, dim3(blocks_for(n)), dim3(kFlat), 0, s, rec, n, n_gb, t.size);
+int launch_graft_sizes(const svo_grafting *rec, DevCount n, uint32_t n_gb, const TreeScratch &t, hipStream_t s) {
+  if (!n.v) return 0;
+  hipLaunchKernelGGL(graft_size_kernel, dim3(blocks_for(n.v)), dim3(kFlat), 0, s, rec, n, n_gb, t.size);
   return launch_rc();
 }
 
-int launch_graft_winners(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s) {
+int launch_graft_winners(const PoolView &p, uint32_t n, DevCount got, const TreeScratch &t, hipStream_t s) {
   (void)p;
   if (!n) return 0;
   hipLaunchKernelGGL(graft_winners_kernel, dim3(blocks_for(n)), dim3(kFlat), 0, s, t.res, t.start, t.need, n, got,
@@ -948,18 +1024,18 @@ int launch_graft_winners(const PoolView &p, uint32_t n, uint32_t got, const Tree
   return launch_rc();
 }
 
-int launch_graft_roots(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s) {
-  if (!n) return 0;
-  hipLaunchKernelGGL(graft_roots_kernel, dim3(blocks_for(n)), dim3(kFlat), 0, s, p, t.res, t.start, t.need, n, got,
+int launch_graft_roots(const PoolView &p, DevCount n, DevCount got, const TreeScratch &t, hipStream_t s) {
+  if (!n.v) return 0;
+  hipLaunchKernelGGL(graft_roots_kernel, dim3(blocks_for(n.v)), dim3(kFlat), 0, s, p, t.res, t.start, t.need, n, got,
                      t.keys, t.vals, t.table_mask, t.front[0], t.counters, t.front_capacity);
   return launch_rc();
 }
 
-int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, uint32_t total_need,
+int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, DevCount total_need,
                        const svo_block *gb, const TreeScratch &t, hipStream_t s) {
   if (!n) return 0;
-  if (total_need) {
-    const uint32_t blocks = (uint32_t)((8ull * total_need + kFlat - 1) / kFlat);
+  if (total_need.v) {
+    const uint32_t blocks = (uint32_t)((8ull * total_need.v + kFlat - 1) / kFlat);
     hipLaunchKernelGGL(graft_place_kernel, dim3(blocks), dim3(kFlat), 0, s, p, rec, gb, t.res, t.start, t.where,
                        t.number, t.parent, t.list, n, total_need);
   }
@@ -968,9 +1044,9 @@ int launch_graft_place(const PoolView &p, const svo_grafting *rec, uint32_t n, u
   return launch_rc();
 }
 
-int launch_graft_release(const PoolView &p, uint32_t n, uint32_t got, const TreeScratch &t, hipStream_t s) {
-  if (!got) return 0;
-  hipLaunchKernelGGL(graft_release_kernel, dim3(blocks_for(got)), dim3(kFlat), 0, s, p, t.res, t.start, n, t.list,
+int launch_graft_release(const PoolView &p, uint32_t n, DevCount got, const TreeScratch &t, hipStream_t s) {
+  if (!got.v) return 0;
+  hipLaunchKernelGGL(graft_release_kernel, dim3(blocks_for(got.v)), dim3(kFlat), 0, s, p, t.res, t.start, n, t.list,
                      got);
   return launch_rc();
 }
