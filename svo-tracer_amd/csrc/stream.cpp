@@ -201,6 +201,51 @@ int check_pool(const PoolBufs &b, const char *fn) {
   return SVO_OK;
 }
 
+// Requests the update reads at most: the ChildRequests records (capped by MaxChildRequestId).
+uint32_t stored_requests(const PoolBufs &b, uint32_t max_child_request_id) {
+  if (!b.requests || !b.request_id || b.request_id->bytes < 4) return 0;
+  uint32_t stored = (uint32_t)std::min<size_t>(b.requests->bytes / sizeof(svo_child_request), 0xffffffffu);
+  return max_child_request_id ? std::min(stored, max_child_request_id) : stored;
+}
+
+// The first-request table of the pool (u64 per block); a table created here starts cleared,
+// at generation 0.
+int first_table(Context *ctx, Stream &st, uint32_t n_blocks, unsigned long long **first) {
+  const std::shared_ptr<DeviceBuffer> before = ctx->stream_scratch[kSbFirst];
+  void *p;
+  int rc = st.grow(kSbFirst, (size_t)n_blocks * 8, &p);
+  if (rc) return rc;
+  *first = (unsigned long long *)p;
+  if (ctx->stream_scratch[kSbFirst] == before) return SVO_OK;
+  ctx->stream_gen = 0;
+  return st.launch(tree_fill_u64(*first, 0ull, ctx->stream_scratch[kSbFirst]->bytes / 8, ctx->stream), "fill");
+}
+
+// Everything an update of this pool and request buffer allocates, so that the frame loop does
+// not allocate: the first-request table, the per-request flags and ranks, the brick list,
+// Grafting / GraftingBlocks for max_bricks and the tree-management scratch.
+int reserve_update(Context *ctx, const PoolBufs &b, uint32_t base_depth, uint32_t max_bricks) {
+  Stream st{ctx};
+  const PoolView pv = view_of(b, base_depth, 0);
+  unsigned long long *first;
+  int rc = first_table(ctx, st, pv.n_blocks, &first);
+  void *p;
+  if (!rc) rc = st.grow(kSbState, kShSlots * 4, &p);
+  const uint32_t stored = stored_requests(b, 0);
+  if (!rc && stored) {
+    const uint32_t nb_max = std::min(stored, max_bricks);
+    std::shared_ptr<DeviceBuffer> rec, gb;
+    if (!rc) rc = st.grow(kSbFlags, (size_t)stored * 4, &p);
+    if (!rc) rc = st.grow(kSbRank, (size_t)stored * 4, &p);
+    if (!rc) rc = st.grow(kSbList, (size_t)nb_max * 4, &p);
+    if (!rc) rc = st.grow(kSbCounts, ((size_t)(stored + 1023u) / 1024u + kShSlots) * 4, &p);
+    if (!rc) rc = ensure_named(ctx, SVO_BUF_GRAFTING, (size_t)nb_max * sizeof(svo_grafting), &rec);
+    if (!rc) rc = ensure_named(ctx, SVO_BUF_GRAFTING_BLOCKS, (size_t)nb_max * 8 * sizeof(svo_block), &gb);
+  }
+  if (!rc) rc = reserve_tree(ctx, base_depth, std::min(stored, max_bricks));
+  return rc;
+}
+
 }  // namespace
 
 extern "C" int svo_terrain_create(svo_context *ctx, const svo_geometry *geometry, svo_terrain **out) {
@@ -288,6 +333,7 @@ extern "C" int svo_stream_seed(svo_context *ctx, const svo_terrain *t, uint32_t 
   if (tree_fill_u64(u, kUsageInviolate, dense_groups, s) ||
       tree_fill_u64(u + dense_groups, kUsageFree, n_usages - dense_groups, s))
     return fail(SVO_ERR_HIP, "svo_stream_seed: usages");
+  if ((rc = reserve_update(ctx, b, base_depth, 65536u))) return rc;
   hipError_t e = hipStreamSynchronize(s);
   return e == hipSuccess ? SVO_OK : hip_fail(e, "svo_stream_seed");
 }
@@ -341,20 +387,16 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
 
   // 1. the frame's child requests (the count keeps counting beyond what was stored): at most
   // `stored`, the exact number on the device
-  uint32_t stored = 0;
-  if (b.requests && b.request_id && b.request_id->bytes >= 4) {
-    stored = (uint32_t)std::min<size_t>(b.requests->bytes / sizeof(svo_child_request), 0xffffffffu);
-    if (o->max_child_request_id) stored = std::min(stored, o->max_child_request_id);
-    if ((rc = st.launch(launch_request_count((const uint32_t *)b.request_id->ptr, stored, state, s), "request count")))
-      return rc;
-  }
+  const uint32_t stored = stored_requests(b, o->max_child_request_id);
+  if (stored &&
+      (rc = st.launch(launch_request_count((const uint32_t *)b.request_id->ptr, stored, state, s), "request count")))
+    return rc;
   // 2. distinct requesting blocks that can take a child level, ranked in request order
   if (stored) {
-    const std::shared_ptr<DeviceBuffer> before = ctx->stream_scratch[kSbFirst];
-    if ((rc = st.grow(kSbFirst, (size_t)pv.n_blocks * 8, &p))) return rc;
-    unsigned long long *first = (unsigned long long *)p;
-    // a new table starts at generation 0, below every generation an update uses
-    if (ctx->stream_scratch[kSbFirst] != before || ++ctx->stream_gen == 0) {
+    unsigned long long *first;
+    if ((rc = first_table(ctx, st, pv.n_blocks, &first))) return rc;
+    // every update takes the next generation; entries of earlier ones never match
+    if (++ctx->stream_gen == 0) {
       if ((rc = st.launch(tree_fill_u64(first, 0ull, ctx->stream_scratch[kSbFirst]->bytes / 8, s), "fill"))) return rc;
       ctx->stream_gen = 1;
     }

@@ -198,35 +198,36 @@ __device__ __forceinline__ unsigned long long first_key(uint32_t gen, uint32_t i
   return ((unsigned long long)gen << 32) | (unsigned long long)(kU32Max - i);
 }
 
+// flags[i] = 1 for the requests that take part (0 beyond the stored requests), so that the
+// second pass reads only their records
 __global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_request *req, uint32_t n_max,
                                                             const uint32_t *n_ptr, unsigned long long *first,
-                                                            uint32_t gen, uint32_t n_blocks) {
+                                                            uint32_t gen, uint32_t n_blocks, uint32_t *flags) {
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-  if (i >= n_max || i >= *n_ptr) return;
-  if (request_inserts(req, i, n_blocks)) atomicMax(&first[req[i].address], first_key(gen, i));
+  if (i >= n_max) return;
+  const bool takes_part = i < *n_ptr && request_inserts(req, i, n_blocks);
+  if (takes_part) atomicMax(&first[req[i].address], first_key(gen, i));
+  flags[i] = takes_part ? 1u : 0u;
 }
 
 // 1 when request i is the first for its address and its block can take a child level (the
 // builder refines exactly the blocks below max_depth that can have children); 0 beyond the
 // stored requests.
+// (after request_first_kernel) flags[i] stays 1 only for a taking-part request
 __global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_request *req, uint32_t n_max,
-                                                           const uint32_t *n_ptr, const unsigned long long *first,
-                                                           uint32_t gen, const uint4 *blocks, uint32_t n_blocks,
-                                                           uint32_t max_depth, uint32_t *flags) {
+                                                           const unsigned long long *first, uint32_t gen,
+                                                           const uint4 *blocks, uint32_t max_depth, uint32_t *flags) {
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-  if (i >= n_max) return;
-  bool ok = i < *n_ptr && request_inserts(req, i, n_blocks);
+  if (i >= n_max || flags[i] == 0u) return;
+  const uint32_t a = req[i].address, depth = req[i].depth;
+  bool ok = first[a] == first_key(gen, i) && depth + 1u <= max_depth && depth >= 1;
   if (ok) {
-    const svo_child_request r = req[i];
-    ok = first[r.address] == first_key(gen, i) && (uint32_t)r.depth + 1u <= max_depth && r.depth >= 1;
-    if (ok) {
-      const uint4 blk = blocks[r.address];
-      // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
-      const uint32_t chunk = blk.y & 0xffffu;
-      ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
-    }
+    const uint4 blk = blocks[a];
+    // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
+    const uint32_t chunk = blk.y & 0xffffu;
+    ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
   }
-  flags[i] = ok ? 1u : 0u;
+  if (!ok) flags[i] = 0u;
 }
 
 // Request list[k] becomes graft record k and brick blocks [8k, 8k+8); eight threads a brick.
@@ -674,10 +675,10 @@ int launch_request_unique(const svo_child_request *req, uint32_t n_max, const ui
                           hipStream_t s) {
   if (!n_max) return 0;
   hipLaunchKernelGGL(request_first_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, n, first, gen,
-                     n_blocks);
+                     n_blocks, flags);
   if (rc_of()) return -1;
-  hipLaunchKernelGGL(request_flag_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, n, first, gen,
-                     blocks, n_blocks, max_depth, flags);
+  hipLaunchKernelGGL(request_flag_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, first, gen,
+                     blocks, max_depth, flags);
   return rc_of();
 }
 

@@ -70,6 +70,9 @@ int graft_device_counted(Context *ctx, const svo_update_input_data &in, const sv
                          const uint32_t *n_dev, const svo_block *graft_blocks, uint32_t n_graft_blocks,
                          const uint32_t **counters);
 int stats_from(const uint32_t *counters, svo_tree_stats *out);
+// Allocates the tree-management scratch of the context's pool, and what a graft of up to
+// `records` one-group records needs, ahead of the first dispatch.
+int reserve_tree(Context *ctx, uint32_t base_depth, uint32_t records);
 // Garbage collection's reclaim of stale live groups (age > in.offset) in the context's pool.
 int reclaim_device(Context *ctx, const svo_update_input_data &in, uint32_t *reclaimed);
 int trace_frame_impl(Context *ctx, const svo_trace_input &in, const svo_frame_options &o, const FrameBuffers &fb);

@@ -406,6 +406,24 @@ int graft_device_counted(Context *ctx, const svo_update_input_data &in, const sv
   return SVO_OK;
 }
 
+int reserve_tree(Context *ctx, uint32_t base_depth, uint32_t records) {
+  svo_update_input_data in;
+  std::memset(&in, 0, sizeof in);
+  in.base_depth = (uint8_t)base_depth;
+  Op op{ctx};
+  int rc = op.init(pool_from_context(ctx), in);
+  if (rc || !records) return rc;
+  // what a graft of `records` one-group records grows: resolution and winner table, the
+  // per-record and per-group arrays, the collection list
+  if ((rc = op.records(records, records + 1u))) return rc;
+  void *p;
+  for (int k : {kTbSize, kTbWhere, kTbNeed, kTbStart, kTbList})
+    if ((rc = op.grow(k, (size_t)records * 4, &p))) return rc;
+  for (int k : {kTbNumber, kTbParent})
+    if ((rc = op.grow(k, ((size_t)records + 1) * 4, &p))) return rc;
+  return SVO_OK;
+}
+
 int reclaim_device(Context *ctx, const svo_update_input_data &in, uint32_t *reclaimed) {
   Op op{ctx};
   int rc = op.init(pool_from_context(ctx), in);
