@@ -112,27 +112,47 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, 
   size_t bases_bytes = ((size_t)n_bases * 2 + 3) & ~(size_t)3;
   *bases_in_lds = bases_bytes <= 32 * 1024;
   if (!*bases_in_lds) bases_bytes = 0;
-  // camera records, then the table of ticked usage groups
-  bases_bytes += extra_lds + (size_t)kTickedSlots * 4;
+  bases_bytes += extra_lds;  // camera records
   // the short stack: per level and lane a block address, its Child field (u32 each) and its
   // chunk (u16)
   const auto stack_bytes = [&](uint32_t n) { return ((size_t)n * wg * kStackEntryBytes + 15) & ~(size_t)15; };
   while (levels > 1 && bases_bytes + stack_bytes(levels) > lds_limit) levels--;
-  size_t lds = bases_bytes + stack_bytes(levels);
-  if (lds > lds_limit) return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup too large for LDS");
+  if (bases_bytes + stack_bytes(levels) > lds_limit)
+    return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: workgroup too large for LDS");
   // register budget: 5 waves per SIMD (96 VGPRs) keeps the step loop free of spills
   const int waves = o.waves_per_simd ? (int)o.waves_per_simd : 5;
   if (waves < 4 || waves > 6)
     return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: waves_per_simd must be 0 (default, 5), 4, 5 or 6");
-  int per_cu = 0;
-  if (trace_occupancy(wg, waves, lds, &per_cu) != 0 || per_cu <= 0)
+  // the work-groups a CU holds by registers alone; unless the caller fixed the stack depth,
+  // the stack keeps that many (a pop below the stack walks down from its deepest level, so
+  // every depth is correct; a deep stack that halves the CU's waves is not worth it: a
+  // streamed pool, whose depth is not known, would otherwise get 19 levels and 2 work-groups)
+  int by_regs = 0, per_cu = 0;
+  if (trace_occupancy(wg, waves, bases_bytes + stack_bytes(1), &by_regs) != 0 || by_regs <= 0)
     return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");
+  for (;;) {
+    if (trace_occupancy(wg, waves, bases_bytes + stack_bytes(levels), &per_cu) != 0 || per_cu <= 0)
+      return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");
+    if (o.stack_levels || per_cu >= by_regs || levels == 1) break;
+    levels--;
+  }
+  size_t lds = bases_bytes + stack_bytes(levels);
+  // the largest table of ticked usage groups that keeps per_cu work-groups on a CU
+  uint32_t slots = 0;
+  for (uint32_t t = kTickedSlots; t >= 64u && !slots; t >>= 1) {
+    int with = 0;
+    if (lds + (size_t)t * 4 <= lds_limit && trace_occupancy(wg, waves, lds + (size_t)t * 4, &with) == 0 &&
+        with >= per_cu)
+      slots = t;
+  }
+  lds += (size_t)slots * 4;
   int grid = per_cu * std::max(1, ctx->cu_count);
   grid = std::max(8, (grid / 8) * 8);
   shape->workgroup = wg;
   shape->waves = waves;
   shape->grid = grid;
   shape->lds_bytes = lds;
+  shape->ticked_slots = slots;
   *stack_levels = levels;
   return SVO_OK;
 }
@@ -309,6 +329,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
     return rc;
   p.bases_in_lds = bases_in_lds ? 1u : 0u;
   p.stack_levels = levels;
+  p.ticked_slots = plan->shape.ticked_slots;
 
   CompactParams &c = plan->compact;
   std::memset(&c, 0, sizeof c);

@@ -14,8 +14,9 @@ constexpr uint32_t kCounterStride = 32;
 // LDS bytes per short-stack entry of the trace kernel: block address and Child (u32 each),
 // chunk (u16)
 constexpr uint32_t kStackEntryBytes = 10;
-// Slots of the trace kernel's per-workgroup LDS table of usage groups already ticked in the
-// launch (direct-mapped on the group index, u32 each)
+// Most slots of the trace kernel's per-workgroup LDS table of usage groups already ticked in
+// the launch (direct-mapped on the group index, u32 each); a launch uses a power of two up to
+// this many, or none, so that the table never lowers the work-groups a CU holds
 constexpr uint32_t kTickedSlots = 1024;
 
 // One camera of a batched frame (96 bytes; read into LDS by the trace kernel).
@@ -70,6 +71,7 @@ struct FrameParams {
   uint32_t max_layers;        // 1..4
   // tuning
   uint32_t stack_levels;
+  uint32_t ticked_slots;  // slots of the LDS table of ticked usage groups (0: no table)
   uint32_t refill;
   uint32_t chunk;  // work items a wave claims per atomic (multiple of 64)
   // tile schedule (may be null): position k of a band's work range holds tile order[k]; a
@@ -83,6 +85,7 @@ struct LaunchShape {
   int waves;  // register budget: waves per SIMD the kernel is compiled for (6)
   int grid;
   size_t lds_bytes;
+  uint32_t ticked_slots;  // the ticked-group table within lds_bytes
 };
 
 struct CompactParams {

@@ -54,7 +54,7 @@ trace_frame_kernel(const FrameParams p) {
   uint32_t *s_cam = smem + bases_words;
   const uint32_t cam_words = p.n_cams * kCamWords;
   uint32_t *s_ticked = s_cam + cam_words;
-  uint32_t *s_stack = s_ticked + kTickedSlots;
+  uint32_t *s_stack = s_ticked + p.ticked_slots;
   if (p.bases_in_lds) {
     for (uint32_t i = threadIdx.x; i < p.n_bases; i += WG) s_bases[i] = p.bases[i];
   }
@@ -62,7 +62,7 @@ trace_frame_kernel(const FrameParams p) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(p.cams);
     for (uint32_t i = threadIdx.x; i < cam_words; i += WG) s_cam[i] = src[i];
   }
-  for (uint32_t i = threadIdx.x; i < kTickedSlots; i += WG) s_ticked[i] = kU32Max;
+  for (uint32_t i = threadIdx.x; i < p.ticked_slots; i += WG) s_ticked[i] = kU32Max;
   __syncthreads();
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -332,9 +332,9 @@ trace_frame_kernel(const FrameParams p) {
         last_group = g;
         resolve_usage();
         if (g < p.n_usages) {
-          const uint32_t slot = g & (kTickedSlots - 1u);
-          if (s_ticked[slot] != g) {
-            s_ticked[slot] = g;
+          const uint32_t slot = g & (p.ticked_slots - 1u);
+          if (p.ticked_slots == 0u || s_ticked[slot] != g) {
+            if (p.ticked_slots) s_ticked[slot] = g;
             pend_g = g;
             pend_t = p.usage16[4u * (size_t)g];
           }
