@@ -42,6 +42,7 @@ enum : uint32_t {
   kTcEvents = 2,      // multi-parent unlink events
   kTcOverflow = 3,    // frontier overflow flag
   kTcTotal = 4,       // scan totals read by the host
+  kTcFront2 = 5,      // third frontier count (level L reads count L % 3, appends to (L+1) % 3)
   kTcApplied = 8,     // statistics (svo_tree_stats order)
   kTcSkipped = 9,
   kTcInvalid = 10,
@@ -56,7 +57,13 @@ enum : uint32_t {
   kTcSlots = 64
 };
 // kTcOverflow bits
-enum : uint32_t { kOverflowFront = 1, kOverflowEvents = 2 };
+enum : uint32_t { kOverflowFront = 1, kOverflowEvents = 2, kOverflowWalk = 4 };
+// free-walk levels launched before the host first reads the frontier
+constexpr uint32_t kSpecLevels = 3;
+// The counter slot of frontier count i (0..2) of the free walk's rotation.
+__host__ __device__ inline uint32_t front_slot(uint32_t i) {
+  return i == 0u ? (uint32_t)kTcFront0 : (i == 1u ? (uint32_t)kTcFront1 : (uint32_t)kTcFront2);
+}
 
 // A count a launch needs: the value itself, or (p set) a device location holding it, with v
 // then its host-known upper bound (grids are sized by v; kernels read min(*p, v)).
@@ -107,12 +114,15 @@ int launch_prune_winners(const PoolView &p, const svo_pruning *rec, uint32_t n, 
                          hipStream_t s);
 int launch_prune_roots(const PoolView &p, const svo_pruning *rec, uint32_t n, const TreeScratch &t,
                        hipStream_t s);
-// The whole free walk from the groups queued in front[0], level by level, in one cooperative
-// launch (grid-wide barriers between levels); leaves both frontier counts at zero.
-int launch_free_walk(const PoolView &p, const TreeScratch &t, bool use_stop, hipStream_t s);
-// Multi-parent unlinks queued by the walk, in order; clears the event count and flags more
-// events than the table holds in counters[kTcOverflow].
-int launch_unlink(const PoolView &p, const TreeScratch &t, hipStream_t s);
+// Level `level` of the free walk from the groups queued in front[0] (count kTcFront0): reads
+// front[level % 2] with count front_slot(level % 3), appends to front[(level + 1) % 2] with
+// count front_slot((level + 1) % 3) and clears front_slot((level + 2) % 3) for the level
+// after; a level whose count is 0 does nothing.
+int launch_free_level(const PoolView &p, const TreeScratch &t, uint32_t level, bool use_stop, hipStream_t s);
+// After `levels` levels: multi-parent unlinks queued by the walk, in order; flags in
+// counters[kTcOverflow] more events than the table holds and a walk whose frontier after the
+// last level is not empty; clears the event and frontier counts for the next walk.
+int launch_unlink(const PoolView &p, const TreeScratch &t, uint32_t levels, hipStream_t s);
 int launch_prune_apply(const PoolView &p, const svo_pruning *rec, uint32_t n, const svo_block *surface,
                        const TreeScratch &t, hipStream_t s);
 

@@ -1,10 +1,10 @@
 // Host side of tree management: svo_prune, svo_graft, svo_collect and the Prune / Graft
 // entries of svo_enqueue (KernelName.Prune / KernelName.Graft, ComputeManager.cs:313-318).
 //
-// Every phase runs on the device and sizes itself from device-side counts (the free walk is
-// one cooperative launch over all levels); the host sequences the kernels and reads the
-// counters back once at the end of a dispatch, plus once after each collection pass that may
-// be followed by another.
+// Every phase runs on the device and sizes itself from device-side counts; the host sequences
+// the kernels and reads the counters back once at the end of a dispatch, once per collection
+// pass, and once per free walk that has not ended within its first kSpecLevels levels (each
+// level reads its frontier size on the device, so the first levels are launched unread).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,6 +66,8 @@ int stats_from(const uint32_t *c, svo_tree_stats *out) {
   if (c[kTcOverflow] & kOverflowFront) return fail(SVO_ERR_CAPACITY, "tree: free walk frontier overflow");
   if (c[kTcOverflow] & kOverflowEvents)
     return fail(SVO_ERR_CAPACITY, "tree: more than 4096 multi-parent unlinks in one dispatch");
+  if (c[kTcOverflow] & kOverflowWalk)
+    return fail(SVO_ERR_CAPACITY, "tree: a free walk expected to be empty was not (streamed graft targets with children)");
   if (!out) return SVO_OK;
   std::memset(out, 0, sizeof *out);
   out->applied = c[kTcApplied];
@@ -170,13 +172,32 @@ struct Op {
   int launch(int r, const char *what) { return r ? fail(SVO_ERR_HIP, std::string("tree: launch ") + what) : SVO_OK; }
 
   // Frees the subtrees queued in front[0] (count in its counter), level by level, then
-  // applies the multi-parent unlink events in order.  Nothing is read back: an overflow of
-  // the frontier or of the event table is flagged on the device and reported by finish().
-  int free_walk(bool use_stop) {
-    int rc = launch(launch_free_walk(pv, ts, use_stop, ctx->stream), "free walk");
+  // applies the multi-parent unlink events in order.  The first kSpecLevels levels are
+  // launched without reading anything back; with `finish_walk` the host then reads the
+  // frontier and adds levels until it is empty.  Without it (a walk known to end within those
+  // levels) an unfinished walk is flagged on the device, as are overflows of the frontier or
+  // of the event table, and finish() reports them.
+  int free_walk(bool use_stop, bool finish_walk = true) {
+    int rc = walk_levels(use_stop, 0, kSpecLevels);
     if (rc) return rc;
-    return launch(launch_unlink(pv, ts, ctx->stream), "unlink");
+    uint32_t level = kSpecLevels;
+    if (finish_walk) {
+      for (; level < kMaxFreeLevels; level++) {
+        if ((rc = read(front_slot(level % 3u), 1))) return rc;
+        if (host[front_slot(level % 3u)] == 0) break;
+        if ((rc = walk_levels(use_stop, level, level + 1u))) return rc;
+      }
+    }
+    return end_walk(level);
   }
+  int walk_levels(bool use_stop, uint32_t from, uint32_t to) {
+    for (uint32_t level = from; level < to; level++) {
+      int rc = launch(launch_free_level(pv, ts, level, use_stop, ctx->stream), "free level");
+      if (rc) return rc;
+    }
+    return SVO_OK;
+  }
+  int end_walk(uint32_t levels) { return launch(launch_unlink(pv, ts, levels, ctx->stream), "unlink"); }
 
   // Collection (graft, wavefront.cl:1277-1305): up to `need` groups (at most need_max) into
   // ts.list; the count collected accumulates in counters[kTcGot].  One pass per age
@@ -197,13 +218,20 @@ struct Op {
       if ((rc = launch(launch_collect_limit(ts, need, ctx->stream), "collect limit"))) return rc;
       if ((rc = launch(launch_collect_scatter(pv, ng, offset, false, ts, ctx->stream), "collect scatter"))) return rc;
       if ((rc = launch(launch_collect_claim(pv, need_max, ts, ctx->stream), "collect claim"))) return rc;
-      if ((rc = free_walk(true))) return rc;
-      if ((rc = launch(launch_collect_mark(pv, need_max, pv.now, ts, ctx->stream), "collect mark"))) return rc;
+      // the pass's groups count as collected now (mark and the next pass read kTcFrom / kTcK)
       if ((rc = launch(launch_collect_advance(ts, ctx->stream), "collect advance"))) return rc;
-      if (offset == 0) break;
-      if ((rc = read(kTcGot, 2))) return rc;  // collected so far and (device need) the need
+      if ((rc = walk_levels(true, 0, kSpecLevels))) return rc;
+      // one read: the walk's frontier, the groups collected so far and (device need) the need
+      if ((rc = read(0, kTcSlots))) return rc;
+      uint32_t level = kSpecLevels;
+      for (; level < kMaxFreeLevels && host[front_slot(level % 3u)] != 0; level++) {
+        if ((rc = walk_levels(true, level, level + 1u))) return rc;
+        if ((rc = read(front_slot((level + 1u) % 3u), 1))) return rc;
+      }
+      if ((rc = end_walk(level))) return rc;
+      if ((rc = launch(launch_collect_mark(pv, need_max, pv.now, ts, ctx->stream), "collect mark"))) return rc;
       const uint32_t nd = need.p ? std::min(host[kTcNeed], need.v) : need.v;
-      if (host[kTcGot] >= nd) break;
+      if (offset == 0 || host[kTcGot] >= nd) break;
     }
     return SVO_OK;
   }
@@ -325,7 +353,9 @@ int graft_impl(Context *ctx, const Pool &pool, const svo_update_input_data &in, 
     return rc;
   if ((rc = op.launch(launch_graft_winners(pv, n, got, ts, s), "graft winners"))) return rc;
   if ((rc = op.launch(launch_graft_roots(pv, records, got, ts, s), "graft roots"))) return rc;
-  if ((rc = op.free_walk(true))) return rc;
+  // device records come from the streaming update, whose targets are childless blocks: their
+  // walk is empty, and one that is not is flagged rather than read back and finished
+  if ((rc = op.free_walk(true, !dev_records))) return rc;
   if ((rc = op.launch(launch_graft_place(pv, rec, n, need, gb, ts, s), "graft place"))) return rc;
   if ((rc = op.launch(launch_graft_release(pv, n, got, ts, s), "graft release"))) return rc;
   if (dev_records) return SVO_OK;

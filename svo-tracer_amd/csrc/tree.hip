@@ -8,8 +8,6 @@
 // holding the largest record index).  Collection ranks candidates in ascending group
 // order with an order-preserving count / scan / scatter, one pass per age threshold.
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "svo_device.hpp"
 #include "svo_tree_kernels.hpp"
 
@@ -478,26 +476,17 @@ __device__ void free_level(const PoolView &p, const uint2 *in, uint32_t n, uint2
   }
 }
 
-// Every level in one launch (cooperative: all work-groups are resident).  The frontier size
-// is read after a grid-wide barrier, so every work-group sees the same value and leaves the
-// loop at the same level; the consumed frontier's count is cleared between two barriers,
-// before the next level appends to it.
-__global__ __launch_bounds__(kFlat) void free_walk_kernel(PoolView p, uint2 *f0, uint2 *f1, uint32_t *counters,
-                                                          uint32_t cap, uint2 *events, uint32_t ev_cap,
-                                                          const uint32_t *stop) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-  uint32_t in = 0;
-  for (uint32_t level = 0; level < kMaxFreeLevels; level++) {
-    const uint32_t c = __hip_atomic_load(&counters[kTcFront0 + in], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t n = c < cap ? c : cap;
-    if (n == 0) break;
-    free_level(p, in ? f1 : f0, n, in ? f0 : f1, &counters[kTcFront0 + 1u - in], cap, &counters[kTcOverflow], events,
-               &counters[kTcEvents], ev_cap, stop);
-    grid.sync();
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kTcFront0 + in] = 0;
-    grid.sync();
-    in ^= 1u;
-  }
+__global__ __launch_bounds__(kFlat) void free_level_kernel(PoolView p, const uint2 *in, uint2 *out,
+                                                           uint32_t *counters, uint32_t c_in, uint32_t c_out,
+                                                           uint32_t c_clear, uint32_t cap, uint2 *events,
+                                                           uint32_t ev_cap, const uint32_t *stop) {
+  // the slot cleared here is neither read nor appended to by this level
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[front_slot(c_clear)] = 0;
+  const uint32_t c = counters[front_slot(c_in)];
+  const uint32_t n = c < cap ? c : cap;
+  if (n == 0) return;
+  free_level(p, in, n, out, &counters[front_slot(c_out)], cap, &counters[kTcOverflow], events, &counters[kTcEvents],
+             ev_cap, stop);
 }
 
 // Multi-parent unlink (helpDereference, wavefront.cl:1105-1148, as intended), applied in
@@ -531,7 +520,7 @@ __device__ void unlink_parent(const PoolView &p, uint32_t g, uint32_t pb) {
 }
 
 __global__ __launch_bounds__(kScanWg) void unlink_kernel(PoolView p, const uint2 *events, uint32_t cap,
-                                                         uint32_t *counters) {
+                                                         uint32_t levels, uint32_t *counters) {
   __shared__ unsigned long long s[kUnlinkSortMax];
   const uint32_t n = counters[kTcEvents] < cap ? counters[kTcEvents] : cap;
   uint32_t m = 2;
@@ -565,9 +554,11 @@ __global__ __launch_bounds__(kScanWg) void unlink_kernel(PoolView p, const uint2
     counters[kTcUnlinks] += done;
     // the next walk starts with no events and empty frontiers
     if (counters[kTcEvents] > cap) counters[kTcOverflow] |= kOverflowEvents;
+    if (counters[front_slot(levels % 3u)] != 0u) counters[kTcOverflow] |= kOverflowWalk;
     counters[kTcEvents] = 0;
     counters[kTcFront0] = 0;
     counters[kTcFront1] = 0;
+    counters[kTcFront2] = 0;
   }
 }
 
@@ -922,28 +913,16 @@ int launch_prune_roots(const PoolView &p, const svo_pruning *rec, uint32_t n, co
   return launch_rc();
 }
 
-int launch_free_walk(const PoolView &p, const TreeScratch &t, bool use_stop, hipStream_t s) {
-  // a cooperative grid must be resident at once: at most two work-groups per CU of the
-  // current device (the walk is a grid-stride loop, so any grid size is correct)
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, free_walk_kernel, kFlat, 0) != hipSuccess)
-    return -1;
-  const uint32_t grid = (uint32_t)std::max(1, std::min(per_cu, 2) * cus);
-  PoolView pv = p;
-  uint2 *f0 = t.front[0], *f1 = t.front[1], *events = t.events;
-  uint32_t *counters = t.counters, cap = t.front_capacity, ev_cap = t.event_capacity;
-  const uint32_t *stop = use_stop ? t.claimed : nullptr;
-  void *args[] = {&pv, &f0, &f1, &counters, &cap, &events, &ev_cap, &stop};
-  if (hipLaunchCooperativeKernel(reinterpret_cast<const void *>(free_walk_kernel), dim3(grid), dim3(kFlat), args, 0,
-                                 s) != hipSuccess)
-    return -1;
+int launch_free_level(const PoolView &p, const TreeScratch &t, uint32_t level, bool use_stop, hipStream_t s) {
+  const uint32_t grid = 1024;  // grid-stride over the device-side count
+  hipLaunchKernelGGL(free_level_kernel, dim3(grid), dim3(kFlat), 0, s, p, t.front[level & 1u], t.front[(level + 1u) & 1u],
+                     t.counters, level % 3u, (level + 1u) % 3u, (level + 2u) % 3u, t.front_capacity, t.events,
+                     t.event_capacity, use_stop ? t.claimed : nullptr);
   return launch_rc();
 }
 
-int launch_unlink(const PoolView &p, const TreeScratch &t, hipStream_t s) {
-  hipLaunchKernelGGL(unlink_kernel, dim3(1), dim3(kScanWg), 0, s, p, t.events, t.event_capacity, t.counters);
+int launch_unlink(const PoolView &p, const TreeScratch &t, uint32_t levels, hipStream_t s) {
+  hipLaunchKernelGGL(unlink_kernel, dim3(1), dim3(kScanWg), 0, s, p, t.events, t.event_capacity, levels, t.counters);
   return launch_rc();
 }
 
