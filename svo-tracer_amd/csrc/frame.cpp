@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -319,7 +320,29 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
       std::memcpy(sc.sched_ranges, p.range_begin, sizeof p.range_begin);
     }
     p.tile_cost = (uint32_t *)sc.tile_cost->ptr;
-    p.order = (const uint32_t *)sc.tile_order->ptr;
+    // costs measured from another view predict this frame's tiles poorly (the heavy tiles move
+    // with the view): the order is used only when every camera moved less than 0.05 of the
+    // volume and turned less than 10 degrees since; the costs are recorded either way
+    std::vector<float> views(6 * (size_t)n_cams);
+    for (uint32_t c = 0; c < n_cams; c++)
+      for (int k = 0; k < 3; k++) {
+        views[6 * c + k] = ins[c].origin[k];
+        views[6 * c + 3 + k] = ins[c].facing[k];
+      }
+    bool coherent = sc.sched_views.size() == views.size();
+    for (uint32_t c = 0; coherent && c < n_cams; c++) {
+      const float *a = &sc.sched_views[6 * c], *b = &views[6 * c];
+      float moved = 0.0f, dot = 0.0f, la = 0.0f, lb = 0.0f;
+      for (int k = 0; k < 3; k++) {
+        moved += (a[k] - b[k]) * (a[k] - b[k]);
+        dot += a[3 + k] * b[3 + k];
+        la += a[3 + k] * a[3 + k];
+        lb += b[3 + k] * b[3 + k];
+      }
+      coherent = moved < 0.05f * 0.05f && dot > 0.9848f * std::sqrt(la * lb);  // cos 10 degrees
+    }
+    sc.sched_views = views;
+    p.order = coherent ? (const uint32_t *)sc.tile_order->ptr : nullptr;
   }
 
   bool bases_in_lds = false;

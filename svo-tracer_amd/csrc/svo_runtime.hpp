@@ -97,6 +97,8 @@ struct FrameScratch {
   // for the tiling (band ranges) they were measured on
   std::shared_ptr<DeviceBuffer> tile_cost, tile_order;
   uint32_t sched_ranges[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // the views (per camera: origin, forward direction) the costs were measured from
+  std::vector<float> sched_views;
 };
 // One launch before a planned frame's trace: clears its counters and totals, writes the first
 // n_inline of host_cams (n_inline <= kInlineCams) to the device records, and orders the
