@@ -314,8 +314,10 @@ enum {
   SVO_FRAME_RASTER_ORDER = 1u << 4   /* take tiles in raster order.  By default the tiles of every band
                                         are taken most expensive first, by the cost measured in the
                                         previous frame of the same tiling (the same cameras repeated, as
-                                        in a graph replay or an optimisation loop, gain most); the
-                                        outputs are the same either way */
+                                        in a graph replay or an optimisation loop, gain most) when every
+                                        camera moved less than 0.25 of the volume and turned less than
+                                        30 degrees since, in raster order otherwise; the outputs are the
+                                        same either way */
 };
 
 /* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block

@@ -320,9 +320,9 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
       std::memcpy(sc.sched_ranges, p.range_begin, sizeof p.range_begin);
     }
     p.tile_cost = (uint32_t *)sc.tile_cost->ptr;
-    // costs measured from another view predict this frame's tiles poorly (the heavy tiles move
-    // with the view): the order is used only when every camera moved less than 0.05 of the
-    // volume and turned less than 10 degrees since; the costs are recorded either way
+    // costs measured from a distant view predict this frame's tiles poorly (the heavy tiles move
+    // with the view): the order is used only when every camera moved less than 0.25 of the
+    // volume and turned less than 30 degrees since; the costs are recorded either way
     std::vector<float> views(6 * (size_t)n_cams);
     for (uint32_t c = 0; c < n_cams; c++)
       for (int k = 0; k < 3; k++) {
@@ -339,7 +339,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
         la += a[3 + k] * a[3 + k];
         lb += b[3 + k] * b[3 + k];
       }
-      coherent = moved < 0.05f * 0.05f && dot > 0.9848f * std::sqrt(la * lb);  // cos 10 degrees
+      coherent = moved < 0.25f * 0.25f && dot > 0.8660f * std::sqrt(la * lb);  // cos 30 degrees
     }
     sc.sched_views = views;
     p.order = coherent ? (const uint32_t *)sc.tile_order->ptr : nullptr;
