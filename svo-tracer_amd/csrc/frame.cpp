@@ -3,7 +3,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -226,6 +225,7 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
 
   FrameParams &p = plan->p;
   std::memset(&p, 0, sizeof p);
+  plan->views = nullptr;
   p.bases = (const uint16_t *)fb.bases.ptr;
   p.n_bases = n_bases;
   p.blocks = (const uint4 *)fb.blocks.ptr;
@@ -310,39 +310,26 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   // idle -- spatial coherence of the 64 cones matters more than lane utilisation
   p.chunk = o.work_chunk ? ((std::min(o.work_chunk, 1u << 20) + 63u) / 64u) * 64u : 64u;
   if (!(o.flags & SVO_FRAME_RASTER_ORDER) && tiles > 0 && !sc.arena) {
-    const DeviceBuffer *old = sc.tile_cost.get();
+    const DeviceBuffer *old_cost = sc.tile_cost.get(), *old_views = sc.tile_views.get();
+    const uint32_t view_words = 8u * n_cams * kViewWords;
     if ((rc = ensure_scratch(sc.tile_cost, (size_t)tiles * 4))) return rc;
     if ((rc = ensure_scratch(sc.tile_order, (size_t)tiles * 4))) return rc;
-    // costs measured on another tiling (or a new buffer) start from zero: raster order
-    if (sc.tile_cost.get() != old || std::memcmp(sc.sched_ranges, p.range_begin, sizeof p.range_begin) != 0) {
+    if ((rc = ensure_scratch(sc.tile_views, (size_t)view_words * 4))) return rc;
+    // costs measured on another tiling (or a new buffer) start from zero, and views recorded
+    // for another camera count (or none) from unrecorded: the first frame runs in raster order
+    if (sc.tile_cost.get() != old_cost || std::memcmp(sc.sched_ranges, p.range_begin, sizeof p.range_begin) != 0) {
       if (launch_zero_u32((uint32_t *)sc.tile_cost->ptr, tiles, ctx->stream) != 0)
         return hip_fail(hipGetLastError(), "trace frame: schedule reset");
       std::memcpy(sc.sched_ranges, p.range_begin, sizeof p.range_begin);
     }
-    p.tile_cost = (uint32_t *)sc.tile_cost->ptr;
-    // costs measured from a distant view predict this frame's tiles poorly (the heavy tiles move
-    // with the view): the order is used only when every camera moved less than 0.25 of the
-    // volume and turned less than 30 degrees since; the costs are recorded either way
-    std::vector<float> views(6 * (size_t)n_cams);
-    for (uint32_t c = 0; c < n_cams; c++)
-      for (int k = 0; k < 3; k++) {
-        views[6 * c + k] = ins[c].origin[k];
-        views[6 * c + 3 + k] = ins[c].facing[k];
-      }
-    bool coherent = sc.sched_views.size() == views.size();
-    for (uint32_t c = 0; coherent && c < n_cams; c++) {
-      const float *a = &sc.sched_views[6 * c], *b = &views[6 * c];
-      float moved = 0.0f, dot = 0.0f, la = 0.0f, lb = 0.0f;
-      for (int k = 0; k < 3; k++) {
-        moved += (a[k] - b[k]) * (a[k] - b[k]);
-        dot += a[3 + k] * b[3 + k];
-        la += a[3 + k] * a[3 + k];
-        lb += b[3 + k] * b[3 + k];
-      }
-      coherent = moved < 0.25f * 0.25f && dot > 0.8660f * std::sqrt(la * lb);  // cos 30 degrees
+    if (sc.tile_views.get() != old_views || sc.sched_cams != n_cams) {
+      if (launch_zero_u32((uint32_t *)sc.tile_views->ptr, view_words, ctx->stream) != 0)
+        return hip_fail(hipGetLastError(), "trace frame: schedule reset");
+      sc.sched_cams = n_cams;
     }
-    sc.sched_views = views;
-    p.order = coherent ? (const uint32_t *)sc.tile_order->ptr : nullptr;
+    p.tile_cost = (uint32_t *)sc.tile_cost->ptr;
+    p.order = (const uint32_t *)sc.tile_order->ptr;
+    plan->views = (float *)sc.tile_views->ptr;
   }
 
   bool bases_in_lds = false;
@@ -389,6 +376,8 @@ int launch_begin(const BatchPlan &plan, const CamRec *host_cams, uint32_t n_inli
   for (uint32_t c = 0; c < n_inline; c++) f.cam[c] = host_cams[c];
   if (plan.p.order) {
     f.bands = 8;
+    f.n_views = plan.n_cams;
+    f.views = plan.views;
     f.cost = plan.p.tile_cost;
     f.order = const_cast<uint32_t *>(plan.p.order);
     std::memcpy(f.range_begin, plan.p.range_begin, sizeof f.range_begin);

@@ -102,19 +102,27 @@ struct CompactParams {
 int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu);
 // What a frame does before its trace launch, in one launch: clear the counters and totals,
 // write up to kInlineCams camera records carried in the argument (more are uploaded by a
-// copy beforehand), and, with bands = 8, order each band's tiles by the cost its waves
-// measured in the previous frame (most expensive first, in power-of-two classes, stable
-// within a class).
+// copy beforehand), and, with bands = 8, build each band's tile order.  A band's tiles are
+// ordered by the cost its waves measured in the previous frame (most expensive first, in
+// power-of-two classes, stable within a class) when every camera moved less than
+// kViewMoved of the volume and turned less than kViewTurnCos (as a cosine) since the views
+// the band recorded last; otherwise the band's order is raster order.  Either way the
+// band then records the frame's views.
 constexpr uint32_t kInlineCams = 4;
+constexpr uint32_t kViewWords = 8;  // origin[3], forward direction[3], recorded flag (1.0f), unused
+constexpr float kViewMoved = 0.25f;
+constexpr float kViewTurnCos = 0.8660f;  // cos 30 degrees
 struct FrameBegin {
   uint32_t *counters;
   unsigned long long *totals;
-  CamRec *cams;     // destination of the inline records
+  CamRec *cams;     // the frame's camera records (destination of the inline records)
   uint32_t n_cams;  // inline records to write, 0..kInlineCams
   uint32_t bands;   // 0 (no tile order) or 8
   uint32_t stage;   // set by the launcher: stage the cost classes in LDS
+  uint32_t n_views; // cameras of the frame (all inline when n_cams > 0)
   const uint32_t *cost;
   uint32_t *order;
+  float *views;     // per band, n_views records of kViewWords floats
   uint32_t range_begin[9];
   CamRec cam[kInlineCams];
 };

@@ -84,6 +84,7 @@ struct BatchPlan {
   bool compaction = false;
   size_t cones = 0;
   uint32_t n_cams = 0;  // camera records uploaded before the kernel (0: single-camera kernel)
+  float *views = nullptr;  // with a tile order: the views each band recorded (FrameBegin::views)
 };
 // Device scratch of frames: one set per context for direct calls, one per captured graph
 // (a graph keeps its own so later frames never move memory a replay writes to).
@@ -94,11 +95,11 @@ struct FrameScratch {
   uint8_t *arena = nullptr;
   size_t arena_bytes = 0, arena_used = 0;
   // tile schedule: the per-tile cost of the last frame and the order built from it, valid
-  // for the tiling (band ranges) they were measured on
-  std::shared_ptr<DeviceBuffer> tile_cost, tile_order;
+  // for the tiling (band ranges) they were measured on, and per band the views it recorded
+  // last (kept on the device, so that graph replays with new poses decide there)
+  std::shared_ptr<DeviceBuffer> tile_cost, tile_order, tile_views;
   uint32_t sched_ranges[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  // the views (per camera: origin, forward direction) the costs were measured from
-  std::vector<float> sched_views;
+  uint32_t sched_cams = 0;  // cameras the recorded views are laid out for
 };
 // One launch before a planned frame's trace: clears its counters and totals, writes the first
 // n_inline of host_cams (n_inline <= kInlineCams) to the device records, and orders the

@@ -681,9 +681,48 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
   }
 }
 
+// True when every camera of the frame is within kViewMoved and kViewTurnCos of the view
+// recorded in `saved` (a band's copy), which then takes the frame's views.  The band's threads
+// share the cameras; the barrier inside __syncthreads_or orders every read of the copy
+// before any write to it.
+__device__ bool views_coherent(const CamRec *cams, uint32_t n, float *saved) {
+  uint32_t far = 0;
+  for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+    const float *a = saved + (size_t)c * kViewWords;
+    const CamRec &r = cams[c];
+    float moved = 0.0f, dot = 0.0f, la = 0.0f, lb = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float d = a[k] - r.origin[k];
+      moved += d * d;
+      dot += a[3 + k] * r.facing[k];
+      la += a[3 + k] * a[3 + k];
+      lb += r.facing[k] * r.facing[k];
+    }
+    const bool near = a[6] == 1.0f && moved < kViewMoved * kViewMoved && dot > kViewTurnCos * sqrtf(la * lb);
+    far |= near ? 0u : 1u;
+  }
+  const bool coherent = __syncthreads_or((int)far) == 0;
+  for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+    float *a = saved + (size_t)c * kViewWords;
+    const CamRec &r = cams[c];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      a[k] = r.origin[k];
+      a[3 + k] = r.facing[k];
+    }
+    a[6] = 1.0f;
+    a[7] = 0.0f;
+  }
+  return coherent;
+}
+
 // The work a frame does before its trace launch, in one launch: work-group 0 clears the
 // frame's counters and totals and writes the camera records that came in the kernel
-// argument; work-groups 0..bands-1 order their band's tiles (above).
+// argument; work-group b < bands builds band b's tile order (cost order when the views
+// are coherent with the band's recorded ones, raster order otherwise).  The camera poses
+// are read from the argument when they came inline (work-group 0 may still be writing the
+// device records), from the device records otherwise (copied before this launch).
 __global__ void __launch_bounds__(kOrderThreads) frame_begin_kernel(FrameBegin f) {
   if (blockIdx.x == 0) {
     const uint32_t i = threadIdx.x;
@@ -693,9 +732,15 @@ __global__ void __launch_bounds__(kOrderThreads) frame_begin_kernel(FrameBegin f
       reinterpret_cast<uint32_t *>(f.cams)[i] = reinterpret_cast<const uint32_t *>(f.cam)[i];
   }
   extern __shared__ uint8_t s_cls[];
-  if (blockIdx.x < f.bands) order_band(f.cost, f.order, f.range_begin[blockIdx.x],
-                                       f.range_begin[blockIdx.x + 1] - f.range_begin[blockIdx.x],
-                                       f.stage ? s_cls : nullptr);
+  if (blockIdx.x < f.bands) {
+    const uint32_t b0 = f.range_begin[blockIdx.x], n = f.range_begin[blockIdx.x + 1] - b0;
+    const CamRec *poses = f.n_cams ? f.cam : f.cams;
+    if (views_coherent(poses, f.n_views, f.views + (size_t)blockIdx.x * f.n_views * kViewWords)) {
+      order_band(f.cost, f.order, b0, n, f.stage ? s_cls : nullptr);
+    } else {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) f.order[b0 + i] = b0 + i;
+    }
+  }
 }
 
 __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
@@ -915,6 +960,7 @@ int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
 
 int launch_frame_begin(const FrameBegin &f, hipStream_t stream) {
   if (f.n_cams > kInlineCams || f.bands > 8u) return -1;
+  if (f.bands && (!f.views || !f.n_views || (f.n_cams && f.n_cams != f.n_views))) return -1;
   // bands of up to kStageTiles tiles stage their cost classes in LDS (one byte a tile)
   constexpr uint32_t kStageTiles = 48u * 1024u;
   uint32_t most = 0;
