@@ -386,12 +386,14 @@ def wavefront_leg(svo, tree, cams, frames: int = 3) -> dict:
                     "iteration, blocking queue-size read-backs), same frames and tree"}
 
 
-def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blocks: int = 26) -> dict:
+def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blocks: int = 26,
+               keep_requests: bool = True) -> dict:
     """Streaming: a 3840x2160 camera flies the seeded 300-frame spline over the fBm terrain
     generated on the device (16384^2 grid) and refined on demand to depth 14 in a fixed pool of
     2^26 blocks; per frame the trace makes child requests and ticks usages, and the streaming
     step grafts up to 65,536 bricks, collects groups unused for 30 frames and compacts when
-    fewer than 10 % of the groups are free."""
+    fewer than 10 % of the groups are free.  keep_requests: the frames keep their requests for
+    the streaming step (SVO_FRAME_KEEP_REQUESTS) instead of writing them to ChildRequests."""
     B = svo.BufferName
     Bd, D, n_groups = 4, 14, 1 << (log2_blocks - 3)
     n = width * height
@@ -419,7 +421,7 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blo
             cam.tick = f + 1
             cam.max_child_request_id = n
             cm.TimerBegin()
-            cm.TraceFrame(cam)
+            cm.TraceFrame(cam, flags=svo.FRAME_KEEP_REQUESTS if keep_requests else 0)
             trace_ms.append(cm.TimerEnd())
             t1 = time.perf_counter()
             stats.append(cm.StreamUpdate(terrain, Bd, D, f + 1, max_bricks=65536, max_child_request_id=n,
@@ -452,7 +454,8 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blo
         "ms_per_step": round(frame_ms, 3),
         "workload": f"{width}x{height}, {frames}-frame seeded spline (seed 3), fBm terrain 16384^2 generated on "
                     f"the device, refined on demand to depth {D}, pool 2^{log2_blocks} blocks, <= 65536 bricks/frame, "
-                    "prune age 30 frames, GC below 10 % free",
+                    "prune age 30 frames, GC below 10 % free, requests "
+                    + ("kept by the frame for the update" if keep_requests else "through ChildRequests"),
         "frames": frames,
         "trace_ms_mean": round(float(np.mean(trace_ms)), 3),
         "update_ms_mean": round(float(np.mean(update_ms)), 3),
@@ -491,6 +494,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
     ap.add_argument("--no-stream", action="store_true", help="skip the streaming leg")
     ap.add_argument("--stream-frames", type=int, default=300)
+    ap.add_argument("--stream-requests", choices=("kept", "buffer"), default="kept",
+                    help="streaming legs: frames keep their child requests for the update, or write ChildRequests")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -677,10 +682,11 @@ def main() -> int:
         log(f"adjoint: {line['adjoint']['value']} Mcones/s, {line['adjoint']['cost_vs_plain_forward']}x plain forward")
     if not args.no_stream:
         log("streaming leg...")
-        line["streaming"] = stream_leg(svo, args.stream_frames)
+        line["streaming"] = stream_leg(svo, args.stream_frames, keep_requests=args.stream_requests == "kept")
         log(f"streaming: {line['streaming']['value']} Mcones/s, update {line['streaming']['update_ms_mean']} ms/frame")
         log("streaming leg with a pool of 2^22 blocks (garbage collection under pressure)...")
-        line["streaming_small_pool"] = stream_leg(svo, 150, log2_blocks=22)
+        line["streaming_small_pool"] = stream_leg(svo, 150, log2_blocks=22,
+                                                  keep_requests=args.stream_requests == "kept")
         sp = line["streaming_small_pool"]
         log(f"streaming, small pool: {sp['value']} Mcones/s, {sp['collected_total']} groups gathered by the grafts' "
             f"collection (free, or unused for 30 frames), {sp['gc_runs']} garbage collections "

@@ -311,13 +311,23 @@ enum {
   SVO_FRAME_NO_XCD_SPLIT = 1u << 2,
   SVO_FRAME_CAMERA_TABLE = 1u << 3, /* accepted for compatibility: camera parameters always come from a
                                        device table (one record per camera) */
-  SVO_FRAME_RASTER_ORDER = 1u << 4   /* take tiles in raster order.  By default the tiles of every band
+  SVO_FRAME_RASTER_ORDER = 1u << 4,  /* take tiles in raster order.  By default the tiles of every band
                                         are taken most expensive first, by the cost measured in the
                                         previous frame of the same tiling (the same cameras repeated, as
                                         in a graph replay or an optimisation loop, gain most) when every
                                         camera moved less than 0.25 of the volume and turned less than
                                         30 degrees since, in raster order otherwise; the outputs are the
                                         same either way */
+  SVO_FRAME_KEEP_REQUESTS = 1u << 5  /* keep the child requests for the next svo_stream_update of the
+                                        context instead of writing them to ChildRequests: the frame
+                                        writes their count to ChildRequestID as usual and keeps, in the
+                                        context, the requesting cones in ascending order (as many as
+                                        ChildRequests would have stored: MaxChildRequestId, and the
+                                        buffer's size when it is bound) with an 8-byte key each;
+                                        ChildRequests is not written.  For svo_trace_frame and
+                                        svo_trace_batch (SVO_ERR_NOT_SUPPORTED elsewhere); needs
+                                        ChildRequestID.  A later direct frame without the flag drops
+                                        the kept requests. */
 };
 
 /* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block
@@ -622,7 +632,9 @@ typedef struct {
 
 /* One streaming step after a traced frame (the WorldManager.UpdateGrafting / RunGraft role,
  * WorldManager.cs:84-103, MainWindow.cs:307-337, fed by requestChild, wavefront.cl:362-375):
- *  1. the stored child requests (ChildRequests, count in ChildRequestID) are read in order;
+ *  1. the stored child requests (ChildRequests, count in ChildRequestID; or, when the context's
+ *     last direct frame was traced with SVO_FRAME_KEEP_REQUESTS, the requests it kept, under the
+ *     same limits: the same result without the 40-byte copies) are read in order;
  *     among those asking for a level (TreeSize >= 1: the cone needs detail below the block),
  *     the first request of every block address whose block is childless, at depth 1 <=
  *     d < max_depth, and can have children (an octant with both air and geometry,

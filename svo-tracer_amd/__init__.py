@@ -136,6 +136,7 @@ FRAME_QUEUES = 2
 FRAME_NO_XCD_SPLIT = 4
 FRAME_CAMERA_TABLE = 8
 FRAME_RASTER_ORDER = 16
+FRAME_KEEP_REQUESTS = 32
 
 GEOMETRY_SPHERES = 0
 GEOMETRY_CUBES = 1

@@ -207,6 +207,9 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
                const svo_frame_options &o, const FrameBuffers &fb, CamRec *host_cams, CamRec *dev_cams,
                const float *lut_dev, const uint32_t *lut_offsets, BatchPlan *plan) {
   size_t N = 0;
+  // planning a direct frame may move the frame scratch: kept requests are valid again only
+  // when this frame keeps its own
+  if (&sc == &ctx->frame_scratch) ctx->kept.valid = false;
   int rc = validate_batch(ins, apertures, n_cams, &N);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
@@ -234,8 +237,17 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   p.n_usages = fb.usages.ptr ? (uint32_t)std::min<size_t>(fb.usages.bytes / 8, 0xffffffffu) : 0u;
   svo_child_request *requests = (svo_child_request *)fb.requests.ptr;
   uint32_t *request_id = (uint32_t *)fb.request_id.ptr;
+  // kept requests stay in the context's frame scratch for svo_stream_update; the capacity is
+  // what ChildRequests would have stored
+  const bool keep = (o.flags & SVO_FRAME_KEEP_REQUESTS) != 0;
+  if (keep && (&sc != &ctx->frame_scratch || sc.arena))
+    return fail(SVO_ERR_NOT_SUPPORTED,
+                "trace frame: SVO_FRAME_KEEP_REQUESTS is for direct frames (svo_trace_frame, svo_trace_batch)");
+  if (keep && !request_id)
+    return fail(SVO_ERR_UNKNOWN_BUFFER, "trace frame: SVO_FRAME_KEEP_REQUESTS needs buffer ChildRequestID");
   const uint32_t req_capacity =
-      requests ? (uint32_t)std::min<size_t>(fb.requests.bytes / sizeof(svo_child_request), in.max_child_request_id) : 0u;
+      requests ? (uint32_t)std::min<size_t>(fb.requests.bytes / sizeof(svo_child_request), in.max_child_request_id)
+               : (keep ? in.max_child_request_id : 0u);
   p.rgba = (float4 *)fb.rgba.ptr;
   p.hit = (uint32_t *)fb.hit.ptr;
   p.visits = (uint32_t *)fb.visits.ptr;
@@ -267,6 +279,14 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
   if (requests || request_id) {
     if ((rc = frame_scratch(sc, sc.requests, N * sizeof(svo_child_request), &sp))) return rc;
     p.req_sparse = (svo_child_request *)sp;
+  }
+  uint32_t *req_cones = nullptr;
+  const uint32_t kept_capacity = (uint32_t)std::min<size_t>(req_capacity, N);
+  if (keep) {
+    if ((rc = frame_scratch(sc, sc.req_keys, N * sizeof(uint2), &sp))) return rc;
+    p.req_keys = (uint2 *)sp;
+    if ((rc = frame_scratch(sc, sc.req_cones, (size_t)std::max(kept_capacity, 1u) * 4, &sp))) return rc;
+    req_cones = (uint32_t *)sp;
   }
   if ((rc = frame_scratch(sc, sc.small, kSmallScratchBytes, &sp))) return rc;
   char *small = (char *)sp;
@@ -357,12 +377,21 @@ int plan_batch(Context *ctx, FrameScratch &sc, const svo_trace_input *ins, const
     }
     c.req_count = request_id;
     c.req_sparse = p.req_sparse;
-    c.requests = requests;
-    c.req_capacity = req_capacity;
+    c.requests = keep ? nullptr : requests;
+    c.req_cones = req_cones;
+    c.req_capacity = keep ? kept_capacity : req_capacity;
     c.totals = p.totals;
   }
   plan->cones = N;
   plan->n_cams = n_cams;
+  // a direct frame replaces the frame scratch the kept requests of an earlier one live in
+  if (&sc == &ctx->frame_scratch) {
+    ctx->kept.valid = keep;
+    ctx->kept.cones = req_cones;
+    ctx->kept.keys = p.req_keys;
+    ctx->kept.records = p.req_sparse;
+    ctx->kept.capacity = kept_capacity;
+  }
   return SVO_OK;
 }
 

@@ -386,8 +386,22 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
   in.offset = o->prune_age;
 
   // 1. the frame's child requests (the count keeps counting beyond what was stored): at most
-  // `stored`, the exact number on the device
-  const uint32_t stored = stored_requests(b, o->max_child_request_id);
+  // `stored`, the exact number on the device.  Requests the last direct frame kept are read
+  // where it kept them, under the same limits as ChildRequests.
+  uint32_t stored = 0;
+  RequestSource src{nullptr, nullptr, nullptr};
+  if (ctx->kept.valid) {
+    if (b.request_id && b.request_id->bytes >= 4) {
+      stored = ctx->kept.capacity;
+      if (o->max_child_request_id) stored = std::min(stored, o->max_child_request_id);
+      if (b.requests)
+        stored = (uint32_t)std::min<size_t>(stored, b.requests->bytes / sizeof(svo_child_request));
+    }
+    src = RequestSource{ctx->kept.records, ctx->kept.cones, ctx->kept.keys};
+  } else {
+    stored = stored_requests(b, o->max_child_request_id);
+    if (stored) src.records = (const svo_child_request *)b.requests->ptr;
+  }
   if (stored &&
       (rc = st.launch(launch_request_count((const uint32_t *)b.request_id->ptr, stored, state, s), "request count")))
     return rc;
@@ -406,7 +420,7 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     uint32_t *rank = (uint32_t *)p;
     if ((rc = st.grow(kSbList, (size_t)std::max(1u, std::min(stored, max_bricks)) * 4, &p))) return rc;
     uint32_t *list = (uint32_t *)p;
-    if ((rc = st.launch(launch_request_unique((const svo_child_request *)b.requests->ptr, stored, state + kShStored,
+    if ((rc = st.launch(launch_request_unique(src, stored, state + kShStored,
                                               first, ctx->stream_gen, (const uint4 *)pv.blocks, pv.n_blocks,
                                               o->max_depth, flags, s),
                         "request unique")))
@@ -425,7 +439,7 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     std::shared_ptr<DeviceBuffer> rec, gb;
     if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING, (size_t)nb_max * sizeof(svo_grafting), &rec))) return rc;
     if ((rc = ensure_named(ctx, SVO_BUF_GRAFTING_BLOCKS, (size_t)nb_max * 8 * sizeof(svo_block), &gb))) return rc;
-    if ((rc = st.launch(launch_bricks(t->view, (const svo_child_request *)b.requests->ptr, list, nb_max,
+    if ((rc = st.launch(launch_bricks(t->view, src, list, nb_max,
                                       state + kShUnique, (svo_grafting *)rec->ptr, (uint4 *)gb->ptr, s),
                         "bricks")))
       return rc;

@@ -182,13 +182,30 @@ __global__ void request_count_kernel(const uint32_t *request_id, uint32_t stored
   counters[1] = c;
 }
 
+// Address, depth and TreeSize of request i (from its key when the frame kept it: 8 bytes
+// instead of the 40-byte record).
+struct RequestKey {
+  uint32_t address, depth;
+  bool tree;
+};
+__device__ __forceinline__ RequestKey request_key(const RequestSource &src, uint32_t i) {
+  if (src.cones) {
+    const uint2 k = src.keys[src.cones[i]];
+    return {k.x, k.y & 0xffu, (k.y >> 8) != 0u};
+  }
+  const svo_child_request &r = src.records[i];
+  return {r.address, r.depth, r.tree_size != 0};
+}
+
 // A request takes part in the de-duplication when its address is in Blocks and its cone needs
 // a level below the block (TreeSize, wavefront.cl:362-375), and it is not right after a
 // taking-part request for the same address (neighbouring cones mostly ask for the same block).
-__device__ __forceinline__ bool request_inserts(const svo_child_request *req, uint32_t i, uint32_t n_blocks) {
-  const uint32_t a = req[i].address;
-  if (a >= n_blocks || req[i].tree_size == 0) return false;
-  return !(i > 0 && req[i - 1].address == a && req[i - 1].tree_size != 0);
+__device__ __forceinline__ bool request_inserts(const RequestSource &src, uint32_t i, uint32_t n_blocks) {
+  const RequestKey k = request_key(src, i);
+  if (k.address >= n_blocks || !k.tree) return false;
+  if (i == 0) return true;
+  const RequestKey q = request_key(src, i - 1);
+  return !(q.address == k.address && q.tree);
 }
 
 // first[a] = (gen << 32) | ~i for the smallest index i of the requests for block a in this
@@ -200,13 +217,13 @@ __device__ __forceinline__ unsigned long long first_key(uint32_t gen, uint32_t i
 
 // flags[i] = 1 for the requests that take part (0 beyond the stored requests), so that the
 // second pass reads only their records
-__global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_request *req, uint32_t n_max,
-                                                            const uint32_t *n_ptr, unsigned long long *first,
-                                                            uint32_t gen, uint32_t n_blocks, uint32_t *flags) {
+__global__ __launch_bounds__(kWg) void request_first_kernel(RequestSource req, uint32_t n_max, const uint32_t *n_ptr,
+                                                            unsigned long long *first, uint32_t gen, uint32_t n_blocks,
+                                                            uint32_t *flags) {
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
   if (i >= n_max) return;
   const bool takes_part = i < *n_ptr && request_inserts(req, i, n_blocks);
-  if (takes_part) atomicMax(&first[req[i].address], first_key(gen, i));
+  if (takes_part) atomicMax(&first[request_key(req, i).address], first_key(gen, i));
   flags[i] = takes_part ? 1u : 0u;
 }
 
@@ -214,12 +231,13 @@ __global__ __launch_bounds__(kWg) void request_first_kernel(const svo_child_requ
 // builder refines exactly the blocks below max_depth that can have children); 0 beyond the
 // stored requests.
 // (after request_first_kernel) flags[i] stays 1 only for a taking-part request
-__global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_request *req, uint32_t n_max,
+__global__ __launch_bounds__(kWg) void request_flag_kernel(RequestSource req, uint32_t n_max,
                                                            const unsigned long long *first, uint32_t gen,
                                                            const uint4 *blocks, uint32_t max_depth, uint32_t *flags) {
   const uint32_t i = blockIdx.x * kWg + threadIdx.x;
   if (i >= n_max || flags[i] == 0u) return;
-  const uint32_t a = req[i].address, depth = req[i].depth;
+  const RequestKey k = request_key(req, i);
+  const uint32_t a = k.address, depth = k.depth;
   bool ok = first[a] == first_key(gen, i) && depth + 1u <= max_depth && depth >= 1;
   if (ok) {
     const uint4 blk = blocks[a];
@@ -231,13 +249,14 @@ __global__ __launch_bounds__(kWg) void request_flag_kernel(const svo_child_reque
 }
 
 // Request list[k] becomes graft record k and brick blocks [8k, 8k+8); eight threads a brick.
-__global__ __launch_bounds__(kWg) void brick_kernel(TerrainView t, const svo_child_request *req, const uint32_t *list,
+__global__ __launch_bounds__(kWg) void brick_kernel(TerrainView t, RequestSource req, const uint32_t *list,
                                                     uint32_t nb_max, const uint32_t *unique, svo_grafting *records,
                                                     uint4 *bricks) {
   const uint32_t tid = blockIdx.x * kWg + threadIdx.x;
   const uint32_t k = tid >> 3, c = tid & 7u;
   if (k >= nb_max || k >= *unique) return;
-  const svo_child_request r = req[list[k]];
+  const uint32_t i = list[k];
+  const svo_child_request r = req.records[req.cones ? req.cones[i] : i];
   const uint32_t D = r.depth, s = 64u - D;
   const uint32_t x = (uint32_t)(r.location[0] >> s), y = (uint32_t)(r.location[1] >> s),
                  z = (uint32_t)(r.location[2] >> s);
@@ -670,7 +689,7 @@ int launch_request_count(const uint32_t *request_id, uint32_t stored, uint32_t *
   return rc_of();
 }
 
-int launch_request_unique(const svo_child_request *req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
+int launch_request_unique(const RequestSource &req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
                           uint32_t gen, const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s) {
   if (!n_max) return 0;
@@ -682,7 +701,7 @@ int launch_request_unique(const svo_child_request *req, uint32_t n_max, const ui
   return rc_of();
 }
 
-int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb_max,
+int launch_bricks(const TerrainView &t, const RequestSource &req, const uint32_t *list, uint32_t nb_max,
                   const uint32_t *unique, svo_grafting *records, uint4 *bricks, hipStream_t s) {
   if (!nb_max) return 0;
   hipLaunchKernelGGL(brick_kernel, dim3((uint32_t)((8ull * nb_max + kWg - 1) / kWg)), dim3(kWg), 0, s, t, req, list,

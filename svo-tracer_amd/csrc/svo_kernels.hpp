@@ -58,6 +58,7 @@ struct FrameParams {
   uint8_t *depth_out;
   uint64_t *loc_out;  // ulong3 with a 32-byte stride
   svo_child_request *req_sparse;
+  uint2 *req_keys;  // (kept requests) per requesting cone: address, depth | TreeSize << 8
   // cameras (one or a batch): the tiles of all cameras form one work space, outputs are
   // indexed by the concatenated cone index
   const CamRec *cams;
@@ -94,7 +95,8 @@ struct CompactParams {
   uint32_t *counts;  // 4 per chunk
   uint32_t *bg_queue, *mat_queue, *bg_count, *mat_count, *req_count;
   const svo_child_request *req_sparse;
-  svo_child_request *requests;
+  svo_child_request *requests;  // the first req_capacity requests are copied here, or
+  uint32_t *req_cones;          // (kept requests) the cone ids of the first req_capacity written here
   uint32_t req_capacity;
   unsigned long long *totals;
 };

@@ -90,6 +90,8 @@ struct BatchPlan {
 // (a graph keeps its own so later frames never move memory a replay writes to).
 struct FrameScratch {
   std::shared_ptr<DeviceBuffer> status, requests, small, counts, cams;
+  // kept requests (SVO_FRAME_KEEP_REQUESTS): 8-byte keys per cone, requesting cone ids in order
+  std::shared_ptr<DeviceBuffer> req_keys, req_cones;
   // With an arena (device memory the caller owns), the frame's scratch is carved from it in
   // 256-byte aligned pieces instead of being allocated, and the tile schedule is not kept.
   uint8_t *arena = nullptr;
@@ -142,6 +144,16 @@ struct Context {
   uint32_t *stream_host = nullptr;
   // generation of the streaming update's first-request table (0: never used)
   uint32_t stream_gen = 0;
+  // The child requests of the last direct frame, when it kept them (SVO_FRAME_KEEP_REQUESTS):
+  // the first `capacity` requesting cone ids in ascending order, a key per requesting cone
+  // and the full records by cone id, all in frame_scratch.
+  struct KeptRequests {
+    bool valid = false;
+    const uint32_t *cones = nullptr;
+    const uint2 *keys = nullptr;
+    const svo_child_request *records = nullptr;
+    uint32_t capacity = 0;
+  } kept;
   // span sums of the multi-work-group scans (tree management, streaming, device builds)
   std::shared_ptr<DeviceBuffer> scan_sums;
   uint32_t *id_host = nullptr;  // pinned word for the queue-counter read-backs (svo_reset_id)

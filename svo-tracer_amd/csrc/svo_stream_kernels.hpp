@@ -34,14 +34,22 @@ int launch_seed(const TerrainView &t, uint16_t *bases, uint32_t B, uint4 *blocks
 // table of mask+1 entries, filled with UINT32_MAX beforehand.
 // counters[0] = min(*request_id, stored), counters[1] = *request_id
 int launch_request_count(const uint32_t *request_id, uint32_t stored, uint32_t *counters, hipStream_t s);
+// The frame's child requests in request order: request i is records[i] (ChildRequests), or,
+// for requests a frame kept, records[cones[i]] with its address, depth and TreeSize also in
+// keys[cones[i]] (x = address, y = depth | TreeSize << 8).
+struct RequestSource {
+  const svo_child_request *records;
+  const uint32_t *cones;  // null: records in request order
+  const uint2 *keys;
+};
 // flags[i] (i < n_max) = 1 for the first request of each block that can take a child level
 // among the *n stored requests; first: u64 per block of the pool, entries of generations
 // below gen are ignored (zero-filled when allocated)
-int launch_request_unique(const svo_child_request *req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
+int launch_request_unique(const RequestSource &req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
                           uint32_t gen, const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s);
 // bricks of the first min(*unique, nb_max) listed requests
-int launch_bricks(const TerrainView &t, const svo_child_request *req, const uint32_t *list, uint32_t nb_max,
+int launch_bricks(const TerrainView &t, const RequestSource &req, const uint32_t *list, uint32_t nb_max,
                   const uint32_t *unique, svo_grafting *records, uint4 *bricks, hipStream_t s);
 // rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.  With
 // a list, the flagged entries of rank below limit are listed in rank order.  span_sums:

@@ -201,6 +201,7 @@ trace_frame_kernel(const FrameParams p) {
       rq.tree_size = tree_size ? 1 : 0;
       for (int k = 0; k < 7; k++) rq.pad1[k] = 0;
       p.req_sparse[ray] = rq;
+      if (p.req_keys) p.req_keys[ray] = make_uint2(addr, (uint32_t)depth | (tree_size ? 256u : 0u));
     }
     resolve_usage();
     tot_vis += nvis;
@@ -863,7 +864,7 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
 // coalesced.  Positions continue the chunk offsets of the scan in ascending cone id.
 __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
     const uint8_t *status, uint32_t n, const uint32_t *offsets, uint32_t *bg_queue, uint32_t *mat_queue,
-    const svo_child_request *req_sparse, svo_child_request *requests, uint32_t req_capacity) {
+    const svo_child_request *req_sparse, svo_child_request *requests, uint32_t *req_cones, uint32_t req_capacity) {
   constexpr uint32_t kRounds = kCompactChunk / kCompactThreads;
   __shared__ uint32_t s_tot[3][kCompactThreads / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -897,6 +898,7 @@ __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
     if (c & 4u) {
       const uint32_t at = pos[2] + rank_below(m2);
       if (requests && at < req_capacity) requests[at] = req_sparse[id];
+      if (req_cones && at < req_capacity) req_cones[at] = id;
     }
     pos[0] += (uint32_t)__popcll(m0);
     pos[1] += (uint32_t)__popcll(m1);
@@ -1030,7 +1032,7 @@ int launch_compaction(const CompactParams &c, hipStream_t stream) {
   hipLaunchKernelGGL(status_scan_kernel, dim3(1), dim3(kCompactThreads), 0, stream, c.counts, chunks, c.bg_count,
                      c.mat_count, c.req_count, c.totals);
   hipLaunchKernelGGL(status_scatter_kernel, dim3(chunks), dim3(kCompactThreads), 0, stream, c.status, c.n, c.counts,
-                     c.bg_queue, c.mat_queue, c.req_sparse, c.requests, c.req_capacity);
+                     c.bg_queue, c.mat_queue, c.req_sparse, c.requests, c.req_cones, c.req_capacity);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

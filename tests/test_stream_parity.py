@@ -2,7 +2,8 @@
 svo_compact) with the oracle: the seed equals TreeBuilder's dense level, every brick equals the
 statically built tree's children of its cell, and after every frame the traced outputs, the
 requests, the graft records and the whole pool are bit-exact against the oracle's streaming
-step run on its own copy of the pool."""
+step run on its own copy of the pool.  Each flight also runs with the frames keeping their
+requests for the update (SVO_FRAME_KEEP_REQUESTS) instead of writing ChildRequests."""
 from __future__ import annotations
 
 import numpy as np
@@ -40,19 +41,24 @@ class GpuStream:
         cm.InitDeviceBuffer(N.Depths, self.n, 1)
         self.terrain = cm.CreateTerrain(geom)
         cm.StreamSeed(self.terrain, B)
+        self.requests_bound = True
 
     def pool(self):
         N, cm, svo = self.svo.BufferName, self.cm, self.svo
         return (cm.ReadBuffer(N.Bases, np.uint16), cm.ReadBuffer(N.Blocks, svo.BLOCK_DTYPE),
                 cm.ReadBuffer(N.Usages, svo.USAGE_DTYPE))
 
-    def trace(self, cam, tick: int) -> dict:
+    def trace(self, cam, tick: int, keep: bool = False, capacity: int = 0) -> dict:
         N, cm, svo, n = self.svo.BufferName, self.cm, self.svo, self.n
         cam.tick = tick
-        cam.max_child_request_id = n
-        cm.TraceFrame(cam, flags=svo.FRAME_STATE)
+        cam.max_child_request_id = capacity or n
+        cm.TraceFrame(cam, flags=svo.FRAME_STATE | (svo.FRAME_KEEP_REQUESTS if keep else 0))
         cm.Wait()
         count = int(cm.ReadBuffer(N.ChildRequestID, np.uint32, 1)[0])
+        if keep or not self.requests_bound:
+            requests = None
+        else:
+            requests = cm.ReadBuffer(N.ChildRequests, svo.CHILD_REQUEST_DTYPE, min(count, capacity or n))
         return {
             "rgba": cm.ReadBuffer(N.OutputImage, np.float32, 4 * n).reshape(n, 4),
             "hit": cm.ReadBuffer(N.HitIndices, np.uint32, n),
@@ -61,12 +67,12 @@ class GpuStream:
             "depth": cm.ReadBuffer(N.Depths, np.uint8, n),
             "location": cm.ReadBuffer(N.Locations, np.uint64, 4 * n).reshape(n, 4)[:, :3],
             "request_count": count,
-            "requests": cm.ReadBuffer(N.ChildRequests, svo.CHILD_REQUEST_DTYPE, min(count, n)),
+            "requests": requests,
         }
 
-    def update(self, tick: int, max_depth: int, **kw) -> dict:
+    def update(self, tick: int, max_depth: int, capacity: int = 0, **kw) -> dict:
         N, cm, svo = self.svo.BufferName, self.cm, self.svo
-        st = cm.StreamUpdate(self.terrain, self.B, max_depth, tick, max_child_request_id=self.n, **kw)
+        st = cm.StreamUpdate(self.terrain, self.B, max_depth, tick, max_child_request_id=capacity or self.n, **kw)
         nb = st["bricks"]
         st["records"] = cm.ReadBuffer(N.Grafting, svo.GRAFTING_DTYPE, nb) if nb else np.zeros(0, svo.GRAFTING_DTYPE)
         st["bricks_array"] = (cm.ReadBuffer(N.GraftingBlocks, svo.BLOCK_DTYPE, 8 * nb) if nb
@@ -99,10 +105,21 @@ def test_seed_matches_the_builder(svo, gpu, B, resolution):
         gs.close()
 
 
-def _stream_and_compare(svo, orc, gpu, geom, B, D, n_groups, cams, hold, frames, **kw):
+def _stream_and_compare(svo, orc, gpu, geom, B, D, n_groups, cams, hold, frames, keep=False, capacity=0,
+                        unbind_requests=False, **kw):
+    """keep: the frames keep their requests for the update; capacity: MaxChildRequestId of the
+    frames and the update (0: one per cone); unbind_requests: ChildRequests is released (kept
+    requests only), so that MaxChildRequestId alone limits them."""
     W, H = cams[0].screen_size[0], cams[0].screen_size[1]
+    cap = capacity or W * H
     st = static_tree(svo, geom, B, D)
     gs = GpuStream(svo, gpu, geom, B, n_groups, W, H)
+    N = svo.BufferName
+    if unbind_requests:
+        gpu.ReleaseBuffer(N.ChildRequests)
+        gs.requests_bound = False
+    elif keep:
+        gpu.InitBuffer(N.ChildRequests, np.full(W * H * 40, 0xA5, np.uint8))
     try:
         bases, blocks, usages = gs.pool()
         pool = orc.TreePool(bases, blocks, usages)
@@ -110,15 +127,16 @@ def _stream_and_compare(svo, orc, gpu, geom, B, D, n_groups, cams, hold, frames,
         history = []
         for f, cam in enumerate(seq):
             tick = f + 1
-            got = gs.trace(cam, tick)
+            got = gs.trace(cam, tick, keep=keep, capacity=capacity)
             cam.tick = tick
-            ref = orc.trace_frame(cam, pool.bases, pool.blocks, pool.usages, request_capacity=W * H, threads=8)
+            ref = orc.trace_frame(cam, pool.bases, pool.blocks, pool.usages, request_capacity=cap, threads=8)
             for key in ("hit", "visits", "status", "rgba", "depth", "location"):
                 _eq(got[key], ref[key], f"frame {f}: {key}")
             assert got["request_count"] == ref["request_count"], f"frame {f}: request count"
-            _eq(got["requests"], ref["requests"], f"frame {f}: requests")
-            upd = gs.update(tick, D, **kw)
-            exp = orc.stream_update(pool, ref["requests"], ref["request_count"], W * H, B, D, tick,
+            if got["requests"] is not None:
+                _eq(got["requests"], ref["requests"], f"frame {f}: requests")
+            upd = gs.update(tick, D, capacity=capacity, **kw)
+            exp = orc.stream_update(pool, ref["requests"], ref["request_count"], cap, B, D, tick,
                                     lambda sel: static_children(st.blocks, B, sel), **kw)
             for key in ("requests", "unique", "bricks", "collected", "gc_runs", "reclaimed", "moved", "free_groups"):
                 assert upd[key] == exp[key], f"frame {f}: {key} gpu={upd[key]} oracle={exp[key]}"
@@ -131,34 +149,90 @@ def _stream_and_compare(svo, orc, gpu, geom, B, D, n_groups, cams, hold, frames,
             if f >= len(cams) and upd["unique"] == 0:
                 break
         check_pool(pool.blocks, pool.usages, B)
+        if keep and not unbind_requests:
+            # the frames kept their requests: ChildRequests was never written
+            assert np.all(gpu.ReadBuffer(N.ChildRequests, np.uint8) == 0xA5)
         return history, got, st
     finally:
         gs.close()
 
 
-def test_stream_matches_oracle_frame_by_frame(svo, orc, gpu):
+@pytest.mark.parametrize("keep", [False, True], ids=["child-requests", "kept-requests"])
+def test_stream_matches_oracle_frame_by_frame(svo, orc, gpu, keep):
     """Fly 10 frames, then hold: bricks, grafts, reclaim and compaction all bit-exact."""
     geom = terrain_geometry(svo, 64)
     B, D = 2, 7
     cams = [svo.scenes.config3_camera(f, 48, 32, frames=10, base_depth=B) for f in range(10)]
     hold = svo.scenes.config3_camera(3, 48, 32, frames=10, base_depth=B)
-    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 560, cams, hold, 30,
+    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 560, cams, hold, 30, keep=keep,
                                         max_bricks=40, prune_age=2, gc_free_percent=50)
     assert sum(h["bricks"] for h in history) > 200
     assert sum(h["reclaimed"] for h in history) > 0 and sum(h["moved"] for h in history) > 0
     assert history[-1]["unique"] == 0
 
 
-def test_stream_brick_cap_and_forced_gc(svo, orc, gpu):
+@pytest.mark.parametrize("keep", [False, True], ids=["child-requests", "kept-requests"])
+def test_stream_brick_cap_and_forced_gc(svo, orc, gpu, keep):
     """Few bricks per frame, garbage collection forced every frame, a deeper tree."""
     geom = terrain_geometry(svo, 128)
     B, D = 2, 8
     cams = [svo.scenes.config3_camera(f, 40, 24, frames=6, base_depth=B) for f in range(6)]
     hold = svo.scenes.config3_camera(5, 40, 24, frames=6, base_depth=B)
-    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 2048, cams, hold, 4,
+    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 2048, cams, hold, 4, keep=keep,
                                         max_bricks=7, prune_age=1, gc_free_percent=0,
                                         flags=svo.STREAM_FORCE_GC)
     assert all(h["gc_runs"] == 1 for h in history) and all(h["bricks"] <= 7 for h in history)
+
+
+@pytest.mark.parametrize("keep,unbind", [(False, False), (True, False), (True, True)],
+                         ids=["child-requests", "kept-requests", "kept-no-buffer"])
+def test_stream_request_capacity(svo, orc, gpu, keep, unbind):
+    """MaxChildRequestId far below the frame's requests: only the first 45 (in cone order) are
+    stored or kept, and the update serves exactly those, with ChildRequests bound or not."""
+    geom = terrain_geometry(svo, 64)
+    B, D = 2, 7
+    cams = [svo.scenes.config3_camera(f, 48, 32, frames=8, base_depth=B) for f in range(8)]
+    hold = svo.scenes.config3_camera(2, 48, 32, frames=8, base_depth=B)
+    history, _, _ = _stream_and_compare(svo, orc, gpu, geom, B, D, 8 ** 3 + 900, cams, hold, 6, keep=keep,
+                                        capacity=45, unbind_requests=unbind, max_bricks=30, prune_age=3,
+                                        gc_free_percent=20)
+    assert any(h["requests"] > 45 for h in history) and sum(h["bricks"] for h in history) > 50
+
+
+def test_kept_requests_rules(svo, orc, gpu):
+    """SVO_FRAME_KEEP_REQUESTS is refused by frame graphs and the allocation-free frame, and a
+    direct frame traced without it drops the kept requests: the update then reads ChildRequests."""
+    geom = terrain_geometry(svo, 64)
+    B, D = 2, 7
+    W, H = 48, 32
+    st = static_tree(svo, geom, B, D)
+    gs = GpuStream(svo, gpu, geom, B, 8 ** 3 + 600, W, H)
+    try:
+        cam_a = svo.scenes.config3_camera(0, W, H, frames=10, base_depth=B)
+        cam_b = svo.scenes.config3_camera(6, W, H, frames=10, base_depth=B)
+        cam_a.max_child_request_id = W * H
+        with pytest.raises(svo.SvoError, match="direct frames"):
+            gpu.CaptureFrame([cam_a], flags=svo.FRAME_KEEP_REQUESTS)
+        with pytest.raises(svo.SvoError, match="direct frames"):
+            gpu.TraceFrameStatic(cam_a, flags=svo.FRAME_KEEP_REQUESTS)
+        bases, blocks, usages = gs.pool()
+        pool = orc.TreePool(bases, blocks, usages)
+        gs.trace(cam_a, 1, keep=True)
+        got = gs.trace(cam_b, 1)  # same tick, another view, requests to ChildRequests
+        cam_b.tick = 1
+        ref = orc.trace_frame(cam_b, pool.bases, pool.blocks, pool.usages, request_capacity=W * H, threads=8)
+        _eq(got["requests"], ref["requests"], "requests")
+        upd = gs.update(1, D, max_bricks=50, prune_age=2, gc_free_percent=0)
+        exp = orc.stream_update(pool, ref["requests"], ref["request_count"], W * H, B, D, 1,
+                                lambda sel: static_children(st.blocks, B, sel), max_bricks=50, prune_age=2,
+                                gc_free_percent=0)
+        assert upd["bricks"] == exp["bricks"] > 0
+        _eq(upd["records"], exp["records"].view(upd["records"].dtype), "graft records")
+        _, gblocks, gusages = gs.pool()
+        _eq(gblocks, pool.blocks, "blocks")
+        _eq(gusages, pool.usages, "usages")
+    finally:
+        gs.close()
 
 
 def test_streamed_frame_converges_to_the_static_tree_gpu(svo, gpu):
