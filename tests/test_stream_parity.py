@@ -273,3 +273,65 @@ def test_streamed_frame_converges_to_the_static_tree_gpu(svo, gpu):
         _eq(got[key], ref[key], key)
     _eq(got["status"] & 0xFD, ref["status"] & 0xFD, "status")
     assert np.count_nonzero(got["hit"] != U32MAX) > n // 4
+
+
+def test_kept_requests_after_reseed(svo, orc, gpu):
+    """The pool is seeded again, larger, between a kept frame and its update (the first-request
+    table grows and starts over): the update serves the kept requests as the oracle's update of
+    the new pool serves that frame's requests."""
+    geom = terrain_geometry(svo, 64)
+    B, D = 2, 7
+    W, H = 48, 32
+    st = static_tree(svo, geom, B, D)
+    gs = GpuStream(svo, gpu, geom, B, 8 ** 3 + 300, W, H)
+    try:
+        cam = svo.scenes.config3_camera(4, W, H, frames=10, base_depth=B)
+        bases, blocks, usages = gs.pool()
+        got = gs.trace(cam, 1, keep=True)
+        cam.tick = 1
+        ref = orc.trace_frame(cam, bases, blocks, usages.copy(), request_capacity=W * H, threads=8)
+        assert got["request_count"] == ref["request_count"] > 0
+        N = svo.BufferName
+        n_groups = 8 ** 3 + 5000
+        gpu.InitDeviceBuffer(N.Blocks, 8 * n_groups, 16)
+        gpu.InitDeviceBuffer(N.Usages, n_groups, 8)
+        gpu.StreamSeed(gs.terrain, B)
+        bases, blocks, usages = gs.pool()
+        pool = orc.TreePool(bases, blocks, usages)
+        kw = dict(max_bricks=60, prune_age=2, gc_free_percent=0)
+        upd = gs.update(1, D, **kw)
+        exp = orc.stream_update(pool, ref["requests"], ref["request_count"], W * H, B, D, 1,
+                                lambda sel: static_children(st.blocks, B, sel), **kw)
+        assert upd["unique"] == exp["unique"] and upd["bricks"] == exp["bricks"] > 0
+        _eq(upd["records"], exp["records"].view(upd["records"].dtype), "graft records")
+        _, gblocks, gusages = gs.pool()
+        _eq(gblocks, pool.blocks, "blocks")
+        _eq(gusages, pool.usages, "usages")
+    finally:
+        gs.close()
+
+
+def test_second_update_of_a_kept_frame(svo, gpu):
+    """Two updates after one frame give the same pools whether the frame kept its requests or
+    wrote ChildRequests (the second update reads the same requests again)."""
+    geom = terrain_geometry(svo, 64)
+    B, D = 2, 7
+    W, H = 48, 32
+    cam = svo.scenes.config3_camera(5, W, H, frames=10, base_depth=B)
+    results = []
+    for keep in (False, True):
+        gs = GpuStream(svo, gpu, geom, B, 8 ** 3 + 800, W, H)
+        try:
+            gs.trace(cam, 1, keep=keep)
+            ups = [gs.update(1, D, max_bricks=25, prune_age=2, gc_free_percent=0) for _ in range(2)]
+            results.append((ups, gs.pool()))
+        finally:
+            gs.close()
+    (u0, p0), (u1, p1) = results
+    for a, b in zip(u0, u1):
+        for key in ("requests", "unique", "bricks", "collected", "free_groups"):
+            assert a[key] == b[key], key
+        _eq(a["records"], b["records"], "graft records")
+    assert u0[0]["bricks"] > 0
+    for a, b, what in zip(p0, p1, ("bases", "blocks", "usages")):
+        _eq(a, b, what)
