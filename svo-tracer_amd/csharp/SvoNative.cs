@@ -118,6 +118,12 @@ namespace SvoTracer.Kernel.Native
 		public uint Flags, MaxTreeDepth, WorkgroupSize, RefillLanes, StackLevels, WorkChunk, WavesPerSimd, Reserved;
 	}
 
+	// NativeFrameOptions.Flags (SVO_FRAME_* of svo.h)
+	public static class FrameFlags
+	{
+		public const uint State = 1, Queues = 2, NoXcdSplit = 4, CameraTable = 8, RasterOrder = 16, KeepRequests = 32;
+	}
+
 	[StructLayout(LayoutKind.Sequential)]
 	public struct NativeCompositeOptions
 	{
