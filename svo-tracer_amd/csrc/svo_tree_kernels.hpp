@@ -58,8 +58,10 @@ enum : uint32_t {
 };
 // kTcOverflow bits
 enum : uint32_t { kOverflowFront = 1, kOverflowEvents = 2, kOverflowWalk = 4 };
-// free-walk levels launched before the host first reads the frontier
+// free-walk levels launched before the host first reads the frontier, and launched between
+// later reads while the frontier is not empty
 constexpr uint32_t kSpecLevels = 3;
+constexpr uint32_t kMoreLevels = 4;
 // The counter slot of frontier count i (0..2) of the free walk's rotation.
 __host__ __device__ inline uint32_t front_slot(uint32_t i) {
   return i == 0u ? (uint32_t)kTcFront0 : (i == 1u ? (uint32_t)kTcFront1 : (uint32_t)kTcFront2);

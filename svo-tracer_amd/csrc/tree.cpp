@@ -182,13 +182,23 @@ struct Op {
     if (rc) return rc;
     uint32_t level = kSpecLevels;
     if (finish_walk) {
-      for (; level < kMaxFreeLevels; level++) {
-        if ((rc = read(front_slot(level % 3u), 1))) return rc;
-        if (host[front_slot(level % 3u)] == 0) break;
-        if ((rc = walk_levels(use_stop, level, level + 1u))) return rc;
-      }
+      if ((rc = read(front_slot(level % 3u), 1))) return rc;
+      if ((rc = walk_on(use_stop, &level))) return rc;
     }
     return end_walk(level);
+  }
+  // After a read of the frontier of `*level`: while it is not empty, the next kMoreLevels
+  // levels go out unread and the frontier after them is read (a walk that outlasts the
+  // unread levels mostly runs several more: one read for them instead of one a level).
+  int walk_on(bool use_stop, uint32_t *level) {
+    while (*level < kMaxFreeLevels && host[front_slot(*level % 3u)] != 0) {
+      const uint32_t to = std::min(*level + kMoreLevels, kMaxFreeLevels);
+      int rc = walk_levels(use_stop, *level, to);
+      if (!rc) rc = read(front_slot(to % 3u), 1);
+      if (rc) return rc;
+      *level = to;
+    }
+    return SVO_OK;
   }
   int walk_levels(bool use_stop, uint32_t from, uint32_t to) {
     for (uint32_t level = from; level < to; level++) {
@@ -224,10 +234,7 @@ struct Op {
       // one read: the walk's frontier, the groups collected so far and (device need) the need
       if ((rc = read(0, kTcSlots))) return rc;
       uint32_t level = kSpecLevels;
-      for (; level < kMaxFreeLevels && host[front_slot(level % 3u)] != 0; level++) {
-        if ((rc = walk_levels(true, level, level + 1u))) return rc;
-        if ((rc = read(front_slot((level + 1u) % 3u), 1))) return rc;
-      }
+      if ((rc = walk_on(true, &level))) return rc;
       if ((rc = end_walk(level))) return rc;
       if ((rc = launch(launch_collect_mark(pv, need_max, pv.now, ts, ctx->stream), "collect mark"))) return rc;
       const uint32_t nd = need.p ? std::min(host[kTcNeed], need.v) : need.v;
