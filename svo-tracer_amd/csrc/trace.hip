@@ -590,8 +590,13 @@ trace_frame_kernel(const FrameParams p) {
 
 // ---- tile schedule ----
 
-// Cost class of a tile: floor(log2(iterations)), clamped to 15; 0 for an unmeasured tile.
-__device__ __forceinline__ uint32_t cost_class(uint32_t c) { return c ? min(15u, 31u - (uint32_t)__clz(c)) : 0u; }
+// Cost class of a tile: floor(log2(iterations)) - 2, clamped to 0..kCostClasses-1 (tiles of
+// fewer than 8 iterations, and unmeasured ones, are class 0; 512 and more, class 7).
+constexpr uint32_t kCostClasses = 8;
+__device__ __forceinline__ uint32_t cost_class(uint32_t c) {
+  const uint32_t l = c ? 31u - (uint32_t)__clz(c) : 0u;
+  return l < 3u ? 0u : min(kCostClasses - 1u, l - 2u);
+}
 
 constexpr uint32_t kOrderThreads = 1024;
 
@@ -604,8 +609,8 @@ constexpr uint32_t kOrderThreads = 1024;
 // pass reads the costs from memory, one load per tile in turn.
 __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order, uint32_t b0, uint32_t n,
                                            uint8_t *cls) {
-  __shared__ uint32_t s_wave[16][kOrderThreads / 64];
-  __shared__ uint32_t s_base[16];
+  __shared__ uint32_t s_wave[kCostClasses][kOrderThreads / 64];
+  __shared__ uint32_t s_base[kCostClasses];
   if (cls) {
     for (uint32_t j0 = 0; j0 < n; j0 += 8u * kOrderThreads) {
       uint32_t c[8];
@@ -625,18 +630,18 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
   const uint32_t per = (n + kOrderThreads - 1) / kOrderThreads;
   const uint32_t i0 = min(n, threadIdx.x * per), i1 = min(n, i0 + per);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t cnt[16];
+  uint32_t cnt[kCostClasses];
 #pragma unroll
-  for (int c = 0; c < 16; c++) cnt[c] = 0;
+  for (int c = 0; c < (int)kCostClasses; c++) cnt[c] = 0;
   for (uint32_t i = i0; i < i1; i++) {
     const uint32_t k = cls ? (uint32_t)cls[i] : cost_class(cost[b0 + i]);
 #pragma unroll
-    for (int c = 0; c < 16; c++) cnt[c] += (k == (uint32_t)c) ? 1u : 0u;
+    for (int c = 0; c < (int)kCostClasses; c++) cnt[c] += (k == (uint32_t)c) ? 1u : 0u;
   }
   // exclusive scan of each class count over the threads, in thread order
-  uint32_t off[16];
+  uint32_t off[kCostClasses];
 #pragma unroll
-  for (int c = 0; c < 16; c++) {
+  for (int c = 0; c < (int)kCostClasses; c++) {
     uint32_t v = cnt[c];
     for (int d = 1; d < 64; d <<= 1) {
       const uint32_t t = __shfl_up(v, d);
@@ -646,7 +651,7 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
     if (lane == 63u) s_wave[c][wave] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 16u) {
+  if (threadIdx.x < kCostClasses) {
     const uint32_t c = threadIdx.x;
     uint32_t run = 0;
     for (uint32_t w = 0; w < kOrderThreads / 64; w++) {
@@ -659,7 +664,7 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t run = 0;
-    for (int c = 15; c >= 0; c--) {
+    for (int c = (int)kCostClasses - 1; c >= 0; c--) {
       const uint32_t t = s_base[c];
       s_base[c] = run;
       run += t;
@@ -667,12 +672,12 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
   }
   __syncthreads();
 #pragma unroll
-  for (int c = 0; c < 16; c++) off[c] += s_base[c] + s_wave[c][wave];
+  for (int c = 0; c < (int)kCostClasses; c++) off[c] += s_base[c] + s_wave[c][wave];
   for (uint32_t i = i0; i < i1; i++) {
     const uint32_t k = cls ? (uint32_t)cls[i] : cost_class(cost[b0 + i]);
     uint32_t pos = 0;
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
+    for (int c = 0; c < (int)kCostClasses; c++) {
       if (k == (uint32_t)c) {
         pos = off[c];
         off[c]++;
