@@ -415,7 +415,7 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blo
         setup_s = time.perf_counter() - t0
         log(f"stream: seed tree ready after {setup_s:.2f} s")
         cams = [svo.scenes.config3_camera(f, width, height, frames=frames, base_depth=Bd) for f in range(frames)]
-        frame_ms, stats = [], []
+        frame_wall_ms, stats = [], []
         # the trace kernel is timed with events on the stream (no synchronisation between the
         # trace and the update, as in a streaming loop); the update ends with the stream idle
         cm.KernelTiming(True)
@@ -428,9 +428,9 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blo
             cm.TraceFrame(cam, flags=svo.FRAME_KEEP_REQUESTS if keep_requests else 0)
             stats.append(cm.StreamUpdate(terrain, Bd, D, f + 1, max_bricks=65536, max_child_request_id=n,
                                          prune_age=30, gc_free_percent=10))
-            frame_ms.append((time.perf_counter() - t1) * 1e3)
+            frame_wall_ms.append((time.perf_counter() - t1) * 1e3)
             if f < 3 or f % 25 == 24:
-                log(f"stream frame {f + 1}: {frame_ms[-1]:.2f} ms, "
+                log(f"stream frame {f + 1}: {frame_wall_ms[-1]:.2f} ms, "
                     f"{stats[-1]['requests']} requests, {stats[-1]['bricks']} bricks, "
                     f"free groups {stats[-1]['free_groups']}")
         cm.Wait()
@@ -463,8 +463,8 @@ def stream_leg(svo, frames: int, width: int = 3840, height: int = 2160, log2_blo
                     + ("kept by the frame for the update" if keep_requests else "through ChildRequests"),
         "frames": frames,
         "trace_ms_mean": round(trace_mean, 3),
-        "update_ms_mean": round(float(np.mean(frame_ms)) - trace_mean, 3),
-        "frame_ms_max": round(float(np.max(frame_ms)), 3),
+        "update_ms_mean": round(float(np.mean(frame_wall_ms)) - trace_mean, 3),
+        "frame_ms_max": round(float(np.max(frame_wall_ms)), 3),
         "timing": "trace_ms_mean: the trace kernel by HIP events; update_ms_mean: the mean frame wall time "
                   "less that (frame begin, compaction, the streaming update, host time)",
         "bricks_total": int(sum(s["bricks"] for s in stats)),

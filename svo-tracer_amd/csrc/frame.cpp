@@ -124,15 +124,16 @@ int choose_launch(Context *ctx, uint32_t B, uint32_t n_bases, size_t extra_lds, 
   if (waves < 4 || waves > 6)
     return fail(SVO_ERR_INVALID_ARGUMENT, "trace frame: waves_per_simd must be 0 (default, 5), 4, 5 or 6");
   // the work-groups a CU holds by registers alone; unless the caller fixed the stack depth,
-  // the stack and a full table of ticked usage groups keep that many (a pop below the stack
-  // walks down from its deepest level, so every depth is correct; a deep stack that halves
-  // the CU's waves is not worth it -- a pool whose depth is not known would otherwise get 19
-  // levels and 2 work-groups -- and neither is one that squeezes the ticked table: 9 levels
-  // with 64 slots took 1.89 ms for a 4K streamed frame, 5 to 8 levels with 512-1024 slots 1.77)
+  // the stack and a ticked-usage table of at least 256 slots keep that many (a pop below the
+  // stack walks down from its deepest level, so every depth is correct; a deep stack that
+  // halves the CU's waves is not worth it -- a pool whose depth is not known would otherwise
+  // get 19 levels and 2 work-groups -- and neither is one that squeezes the table: 9 levels
+  // with 64 slots took 1.89 ms for a 4K streamed frame, 8 levels with 512 slots 1.77; the
+  // 64-camera batch wants the deeper stack, 6 levels with 256 slots beating 5 with 1024)
   int by_regs = 0, per_cu = 0;
   if (trace_occupancy(wg, waves, bases_bytes + stack_bytes(1), &by_regs) != 0 || by_regs <= 0)
     return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");
-  const size_t table_bytes = o.stack_levels ? 0 : (size_t)kTickedSlots * 4;
+  const size_t table_bytes = o.stack_levels ? 0 : (size_t)kMinTickedSlots * 4;
   for (;;) {
     if (trace_occupancy(wg, waves, bases_bytes + stack_bytes(levels) + table_bytes, &per_cu) != 0 || per_cu <= 0)
       return fail(SVO_ERR_HIP, "trace frame: occupancy query failed");

@@ -17,6 +17,7 @@ constexpr uint32_t kStackEntryBytes = 10;
 // Most slots of the trace kernel's per-workgroup LDS table of usage groups already ticked in
 // the launch (direct-mapped on the group index, u32 each); a launch uses a power of two up to
 // this many, or none, so that the table never lowers the work-groups a CU holds
+constexpr uint32_t kMinTickedSlots = 256;  // what the default stack depth leaves room for
 constexpr uint32_t kTickedSlots = 1024;
 
 // One camera of a batched frame (96 bytes; read into LDS by the trace kernel).
