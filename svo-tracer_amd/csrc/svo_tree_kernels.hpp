@@ -54,6 +54,7 @@ enum : uint32_t {
   kTcNeed = 17,       // graft: groups the grafts need (scan total of need)
   kTcK = 18,          // collection: groups this pass claims
   kTcFrom = 19,       // collection: where this pass's groups start in the list
+  kTcMisses = 20,     // resolve: 1 when some record's walk met a childless block
   kTcSlots = 64
 };
 // kTcOverflow bits

@@ -267,6 +267,7 @@ __global__ __launch_bounds__(kFlat) void prune_resolve_kernel(PoolView p, const 
     o = resolve_target(p, r.depth, r.address, locs, n_locs);
   }
   res[i] = o;
+  if (o.status == kResMiss) counters[kTcMisses] = 1u;
   const uint32_t slot = (o.status == kResOk || o.status == kResBase) ? kTcApplied
                         : o.status == kResInvalid                   ? kTcInvalid
                                                                     : kTcSkipped;
@@ -275,12 +276,13 @@ __global__ __launch_bounds__(kFlat) void prune_resolve_kernel(PoolView p, const 
 
 __global__ __launch_bounds__(kFlat) void graft_resolve_kernel(PoolView p, const svo_grafting *rec, uint32_t n,
                                                               const uint32_t *need, const uint64_t *locs,
-                                                              uint32_t n_locs, ResolveOut *res) {
+                                                              uint32_t n_locs, ResolveOut *res, uint32_t *counters) {
   const uint32_t i = blockIdx.x * kFlat + threadIdx.x;
   if (i >= n) return;
   ResolveOut o{kResInvalid, 0u, 0u, 0u};
   if (need[i]) o = resolve_target(p, rec[i].depth, rec[i].graft_address, locs, n_locs);
   res[i] = o;
+  if (o.status == kResMiss) counters[kTcMisses] = 1u;
 }
 
 struct RecordFields {
@@ -297,6 +299,9 @@ struct RecordFields {
 __global__ __launch_bounds__(kScanWg) void request_scan_kernel(PoolView p, RecordFields f, uint32_t n,
                                                                const uint64_t *locs, const ResolveOut *res,
                                                                uint32_t *counters) {
+  // no record missed (every graft of the streaming update addresses its block directly):
+  // nothing to append, and ChildRequestID and the request count stay as they are
+  if (counters[kTcMisses] == 0u) return;
   __shared__ uint32_t s_waves[kScanWg / 64];
   __shared__ uint32_t s_base;
   if (threadIdx.x == 0) s_base = p.request_count ? *p.request_count : 0u;
@@ -877,7 +882,7 @@ int launch_graft_resolve(const PoolView &p, const svo_grafting *rec, uint32_t n,
                          uint32_t n_locs, const TreeScratch &t, hipStream_t s) {
   if (!n) return 0;
   hipLaunchKernelGGL(graft_resolve_kernel, dim3(blocks_for(n)), dim3(kFlat), 0, s, p, rec, n, t.need, locs, n_locs,
-                     t.res);
+                     t.res, t.counters);
   return launch_rc();
 }
 
