@@ -867,9 +867,22 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
 // in 16 rounds of 64 consecutive ids, one per lane: each round's entries of a queue go to
 // consecutive positions (ranks from ballots), so the record reads and the queue writes are
 // coalesced.  Positions continue the chunk offsets of the scan in ascending cone id.
+// fused (at most kFusedChunks chunks): `offsets` holds the chunks' own counts, each work-group
+// sums those before it, and the last one writes the queue counters and the request total.
+constexpr uint32_t kFusedChunks = 1024;
 __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
     const uint8_t *status, uint32_t n, const uint32_t *offsets, uint32_t *bg_queue, uint32_t *mat_queue,
-    const svo_child_request *req_sparse, svo_child_request *requests, uint32_t *req_cones, uint32_t req_capacity) {
+    const svo_child_request *req_sparse, svo_child_request *requests, uint32_t *req_cones, uint32_t req_capacity,
+    uint32_t fused, uint32_t *bg_count, uint32_t *mat_count, uint32_t *req_count, unsigned long long *totals) {
+  uint32_t pre[3];
+  if (fused) {
+    uint32_t v[3] = {0u, 0u, 0u};
+    for (uint32_t c = threadIdx.x; c < blockIdx.x; c += kCompactThreads)
+      for (int q = 0; q < 3; q++) v[q] += offsets[4u * c + q];
+    block_scan3(v, pre);
+  } else {
+    for (int q = 0; q < 3; q++) pre[q] = offsets[4u * blockIdx.x + q];
+  }
   constexpr uint32_t kRounds = kCompactChunk / kCompactThreads;
   __shared__ uint32_t s_tot[3][kCompactThreads / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -890,8 +903,17 @@ __global__ void __launch_bounds__(kCompactThreads) status_scatter_kernel(
   uint32_t pos[3];
 #pragma unroll
   for (uint32_t q = 0; q < 3; q++) {
-    pos[q] = offsets[4u * blockIdx.x + q];
+    pos[q] = pre[q];
     for (uint32_t w = 0; w < wave; w++) pos[q] += s_tot[q][w];
+  }
+  if (fused && blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) {
+    uint32_t all[3] = {pre[0], pre[1], pre[2]};
+    for (uint32_t q = 0; q < 3; q++)
+      for (uint32_t w = 0; w < kCompactThreads / 64; w++) all[q] += s_tot[q][w];
+    if (bg_count) *bg_count = all[0];
+    if (mat_count) *mat_count = all[1];
+    if (req_count) *req_count = all[2];
+    totals[2] = all[2];
   }
 #pragma unroll
   for (uint32_t r = 0; r < kRounds; r++) {
@@ -1034,10 +1056,17 @@ int launch_compaction(const CompactParams &c, hipStream_t stream) {
   const uint32_t chunks = compaction_chunks(c.n);
   if (chunks == 0) return 0;
   hipLaunchKernelGGL(status_count_kernel, dim3(chunks), dim3(kCompactThreads), 0, stream, c.status, c.n, c.counts);
+  if (chunks <= kFusedChunks) {
+    hipLaunchKernelGGL(status_scatter_kernel, dim3(chunks), dim3(kCompactThreads), 0, stream, c.status, c.n, c.counts,
+                       c.bg_queue, c.mat_queue, c.req_sparse, c.requests, c.req_cones, c.req_capacity, 1u, c.bg_count,
+                       c.mat_count, c.req_count, c.totals);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
   hipLaunchKernelGGL(status_scan_kernel, dim3(1), dim3(kCompactThreads), 0, stream, c.counts, chunks, c.bg_count,
                      c.mat_count, c.req_count, c.totals);
   hipLaunchKernelGGL(status_scatter_kernel, dim3(chunks), dim3(kCompactThreads), 0, stream, c.status, c.n, c.counts,
-                     c.bg_queue, c.mat_queue, c.req_sparse, c.requests, c.req_cones, c.req_capacity);
+                     c.bg_queue, c.mat_queue, c.req_sparse, c.requests, c.req_cones, c.req_capacity, 0u, c.bg_count,
+                     c.mat_count, c.req_count, c.totals);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
