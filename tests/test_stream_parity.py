@@ -335,3 +335,34 @@ def test_second_update_of_a_kept_frame(svo, gpu):
     assert u0[0]["bricks"] > 0
     for a, b, what in zip(p0, p1, ("bases", "blocks", "usages")):
         _eq(a, b, what)
+
+
+def test_kept_requests_of_a_batch(svo, gpu):
+    """Two cameras traced in one batch (cone ids run over both): the update serves the batch's
+    kept requests exactly as it serves them through ChildRequests."""
+    geom = terrain_geometry(svo, 64)
+    B, D = 2, 7
+    W, H = 40, 24
+    results = []
+    for keep in (False, True):
+        gs = GpuStream(svo, gpu, geom, B, 8 ** 3 + 800, 2 * W, H)
+        try:
+            cams = [svo.scenes.config3_camera(f, W, H, frames=10, base_depth=B) for f in (2, 7)]
+            for c in cams:
+                c.tick = 1
+                c.max_child_request_id = 2 * W * H
+            gpu.TraceBatch(cams, flags=svo.FRAME_KEEP_REQUESTS if keep else 0)
+            count = int(gpu.ReadBuffer(svo.BufferName.ChildRequestID, np.uint32, 1)[0])
+            ups = [gs.update(1, D, max_bricks=40, prune_age=2, gc_free_percent=0)]
+            results.append((count, ups, gs.pool()))
+        finally:
+            gs.close()
+    (c0, u0, p0), (c1, u1, p1) = results
+    assert c0 == c1 > 0
+    for a, b in zip(u0, u1):
+        for key in ("requests", "unique", "bricks", "collected", "free_groups"):
+            assert a[key] == b[key], key
+        _eq(a["records"], b["records"], "graft records")
+    assert u0[0]["bricks"] > 0
+    for a, b, what in zip(p0, p1, ("bases", "blocks", "usages")):
+        _eq(a, b, what)
