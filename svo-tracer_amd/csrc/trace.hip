@@ -604,29 +604,36 @@ constexpr uint32_t kOrderThreads = 1024;
 // expensive class first, so that the tiles that ran longest in the previous frame start
 // first and the frame does not end waiting on an expensive tile taken late.  Thread t
 // owns a contiguous run of the band's tiles.
-// With cls (LDS of at least n bytes) the band's cost classes are staged there first, eight
-// loads a thread in flight together, and both passes read them from LDS; without it each
-// pass reads the costs from memory, one load per tile in turn.
+// With cls (LDS of at least (n + 1) / 2 bytes) the band's cost classes are staged there
+// first, eight loads a thread in flight together, and both passes read them from LDS; without
+// it each pass reads the costs from memory, one load per tile in turn (bands of 70K tiles, as
+// in a 64-camera batch, took 140 us that way).
 __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order, uint32_t b0, uint32_t n,
                                            uint8_t *cls) {
   __shared__ uint32_t s_wave[kCostClasses][kOrderThreads / 64];
   __shared__ uint32_t s_base[kCostClasses];
+  // staged classes: two tiles a byte (tile i in the low nibble of byte i/2 when i is even),
+  // each byte written whole by one thread
+  const uint32_t n_bytes = (n + 1u) / 2u;
   if (cls) {
-    for (uint32_t j0 = 0; j0 < n; j0 += 8u * kOrderThreads) {
+    for (uint32_t j0 = 0; j0 < n_bytes; j0 += 4u * kOrderThreads) {
       uint32_t c[8];
 #pragma unroll
       for (uint32_t t = 0; t < 8u; t++) {
-        const uint32_t i = j0 + t * kOrderThreads + threadIdx.x;
+        const uint32_t i = 2u * (j0 + (t >> 1) * kOrderThreads + threadIdx.x) + (t & 1u);
         c[t] = i < n ? cost[b0 + i] : 0u;
       }
 #pragma unroll
-      for (uint32_t t = 0; t < 8u; t++) {
-        const uint32_t i = j0 + t * kOrderThreads + threadIdx.x;
-        if (i < n) cls[i] = (uint8_t)cost_class(c[t]);
+      for (uint32_t t = 0; t < 4u; t++) {
+        const uint32_t byte = j0 + t * kOrderThreads + threadIdx.x;
+        if (byte < n_bytes) cls[byte] = (uint8_t)(cost_class(c[2u * t]) | (cost_class(c[2u * t + 1u]) << 4));
       }
     }
     __syncthreads();
   }
+  auto class_of = [&](uint32_t i) -> uint32_t {
+    return cls ? ((uint32_t)cls[i >> 1] >> (4u * (i & 1u))) & 15u : cost_class(cost[b0 + i]);
+  };
   const uint32_t per = (n + kOrderThreads - 1) / kOrderThreads;
   const uint32_t i0 = min(n, threadIdx.x * per), i1 = min(n, i0 + per);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -634,7 +641,7 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
 #pragma unroll
   for (int c = 0; c < (int)kCostClasses; c++) cnt[c] = 0;
   for (uint32_t i = i0; i < i1; i++) {
-    const uint32_t k = cls ? (uint32_t)cls[i] : cost_class(cost[b0 + i]);
+    const uint32_t k = class_of(i);
 #pragma unroll
     for (int c = 0; c < (int)kCostClasses; c++) cnt[c] += (k == (uint32_t)c) ? 1u : 0u;
   }
@@ -674,7 +681,7 @@ __device__ __forceinline__ void order_band(const uint32_t *cost, uint32_t *order
 #pragma unroll
   for (int c = 0; c < (int)kCostClasses; c++) off[c] += s_base[c] + s_wave[c][wave];
   for (uint32_t i = i0; i < i1; i++) {
-    const uint32_t k = cls ? (uint32_t)cls[i] : cost_class(cost[b0 + i]);
+    const uint32_t k = class_of(i);
     uint32_t pos = 0;
 #pragma unroll
     for (int c = 0; c < (int)kCostClasses; c++) {
@@ -990,13 +997,13 @@ int trace_occupancy(int wg, int waves, size_t lds_bytes, int *blocks_per_cu) {
 int launch_frame_begin(const FrameBegin &f, hipStream_t stream) {
   if (f.n_cams > kInlineCams || f.bands > 8u) return -1;
   if (f.bands && (!f.views || !f.n_views || (f.n_cams && f.n_cams != f.n_views))) return -1;
-  // bands of up to kStageTiles tiles stage their cost classes in LDS (one byte a tile)
-  constexpr uint32_t kStageTiles = 48u * 1024u;
+  // bands of up to kStageTiles tiles stage their cost classes in LDS (two tiles a byte)
+  constexpr uint32_t kStageTiles = 96u * 1024u;
   uint32_t most = 0;
   for (uint32_t b = 0; b < f.bands; b++) most = std::max(most, f.range_begin[b + 1] - f.range_begin[b]);
   FrameBegin g = f;
   g.stage = most > 0u && most <= kStageTiles ? 1u : 0u;
-  const size_t lds = g.stage ? (size_t)((most + 15u) & ~15u) : 0;
+  const size_t lds = g.stage ? (size_t)(((most + 1u) / 2u + 15u) & ~15u) : 0;
   hipLaunchKernelGGL(frame_begin_kernel, dim3(std::max(1u, f.bands)), dim3(kOrderThreads), lds, stream, g);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
