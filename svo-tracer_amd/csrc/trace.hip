@@ -840,22 +840,31 @@ __global__ void __launch_bounds__(kCompactThreads) status_count_kernel(const uin
     for (int c = 0; c < 3; c++) counts[4u * blockIdx.x + c] = total[c];
 }
 
-// Single-workgroup exclusive scan of the per-chunk counts; writes the queue counters.
+// Single-workgroup exclusive scan of the per-chunk counts; writes the queue counters.  Thread
+// t sums a contiguous run of chunks, one scan over the threads gives each run its start, and
+// the thread writes its run's prefixes (one barrier round however many chunks).
 __global__ void __launch_bounds__(kCompactThreads) status_scan_kernel(uint32_t *counts, uint32_t chunks,
                                                                        uint32_t *bg_count, uint32_t *mat_count,
                                                                        uint32_t *req_count,
                                                                        unsigned long long *totals) {
-  uint32_t carry[3] = {0u, 0u, 0u};
-  for (uint32_t start = 0; start < chunks; start += kCompactThreads) {
-    const uint32_t i = start + threadIdx.x;
-    uint32_t v[3] = {0u, 0u, 0u};
-    if (i < chunks)
-      for (int c = 0; c < 3; c++) v[c] = counts[4u * i + c];
-    uint32_t total[3];
-    block_scan3(v, total);
-    if (i < chunks)
-      for (int c = 0; c < 3; c++) counts[4u * i + c] = carry[c] + v[c];
-    for (int c = 0; c < 3; c++) carry[c] += total[c];
+  uint4 *cnt = reinterpret_cast<uint4 *>(counts);
+  const uint32_t per = (chunks + kCompactThreads - 1u) / kCompactThreads;
+  const uint32_t c0 = min(chunks, threadIdx.x * per), c1 = min(chunks, c0 + per);
+  uint32_t v[3] = {0u, 0u, 0u};
+  for (uint32_t i = c0; i < c1; i++) {
+    const uint4 t = cnt[i];
+    v[0] += t.x;
+    v[1] += t.y;
+    v[2] += t.z;
+  }
+  uint32_t carry[3];
+  block_scan3(v, carry);
+  for (uint32_t i = c0; i < c1; i++) {
+    const uint4 t = cnt[i];
+    cnt[i] = make_uint4(v[0], v[1], v[2], t.w);
+    v[0] += t.x;
+    v[1] += t.y;
+    v[2] += t.z;
   }
   if (threadIdx.x == 0) {
     if (bg_count) *bg_count = carry[0];
