@@ -33,7 +33,7 @@ struct svo_terrain {
 
 namespace {
 
-enum StreamBuf { kSbFirst, kSbState, kSbFlags, kSbRank, kSbList, kSbCounts, kSbFrom, kSbTo, kSbRemap, kSbCount };
+enum StreamBuf { kSbFirst, kSbState, kSbFlags, kSbList, kSbCounts, kSbFrom, kSbTo, kSbRemap, kSbCount };
 // kShTotal: scan totals of select(); the update's own counters (kSbState): stored requests,
 // all requests, distinct requesting blocks, free groups
 enum : uint32_t { kShTotal = 0, kShStored = 0, kShRequests = 1, kShUnique = 2, kShFree = 3, kShSlots = 16 };
@@ -222,7 +222,7 @@ int first_table(Context *ctx, Stream &st, uint32_t n_blocks, unsigned long long 
 }
 
 // Everything an update of this pool and request buffer allocates, so that the frame loop does
-// not allocate: the first-request table, the per-request flags and ranks, the brick list,
+// not allocate: the first-request table, the per-request flags, the brick list,
 // Grafting / GraftingBlocks for max_bricks and the tree-management scratch.
 int reserve_update(Context *ctx, const PoolBufs &b, uint32_t base_depth, uint32_t max_bricks) {
   Stream st{ctx};
@@ -236,7 +236,6 @@ int reserve_update(Context *ctx, const PoolBufs &b, uint32_t base_depth, uint32_
     const uint32_t nb_max = std::min(stored, max_bricks);
     std::shared_ptr<DeviceBuffer> rec, gb;
     if (!rc) rc = st.grow(kSbFlags, (size_t)stored * 4, &p);
-    if (!rc) rc = st.grow(kSbRank, (size_t)stored * 4, &p);
     if (!rc) rc = st.grow(kSbList, (size_t)nb_max * 4, &p);
     if (!rc) rc = st.grow(kSbCounts, ((size_t)(stored + 1023u) / 1024u + kShSlots) * 4, &p);
     if (!rc) rc = ensure_named(ctx, SVO_BUF_GRAFTING, (size_t)nb_max * sizeof(svo_grafting), &rec);
@@ -416,8 +415,6 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     }
     if ((rc = st.grow(kSbFlags, (size_t)stored * 4, &p))) return rc;
     uint32_t *flags = (uint32_t *)p;
-    if ((rc = st.grow(kSbRank, (size_t)stored * 4, &p))) return rc;
-    uint32_t *rank = (uint32_t *)p;
     if ((rc = st.grow(kSbList, (size_t)std::max(1u, std::min(stored, max_bricks)) * 4, &p))) return rc;
     uint32_t *list = (uint32_t *)p;
     if ((rc = st.launch(launch_request_unique(src, stored, state + kShStored,
@@ -429,7 +426,8 @@ extern "C" int svo_stream_update(svo_context *ctx, const svo_terrain *t, const s
     if ((rc = st.grow(kSbCounts, ((size_t)chunks + kShSlots) * 4, &p))) return rc;
     uint32_t *sums = nullptr;
     if ((rc = context_scan_sums(ctx, scan_span_sums_bytes(chunks), &sums))) return rc;
-    if ((rc = st.launch(launch_rank(flags, stored, (uint32_t *)p, rank, state + kShUnique, sums, s, list, max_bricks),
+    if ((rc = st.launch(launch_rank(flags, stored, (uint32_t *)p, nullptr, state + kShUnique, sums, s, list,
+                                    max_bricks, state + kShStored),
                         "rank")))
       return rc;
     // 3. records and bricks of the first min(unique, max_bricks) into Grafting /

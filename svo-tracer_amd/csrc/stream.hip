@@ -215,37 +215,46 @@ __device__ __forceinline__ unsigned long long first_key(uint32_t gen, uint32_t i
   return ((unsigned long long)gen << 32) | (unsigned long long)(kU32Max - i);
 }
 
-// flags[i] = 1 for the requests that take part (0 beyond the stored requests), so that the
-// second pass reads only their records
+// flags[i] = 1 for the stored requests (i < *n_ptr) that take part, so that the second pass
+// reads only their records; flags beyond the stored requests are not written.  Grid-stride
+// over the device-side count: the grid covers at most kRequestGrid work-groups.
+constexpr uint32_t kRequestGrid = 2048;
+__device__ __forceinline__ uint32_t stored_count(const uint32_t *n_ptr, uint32_t n_max) {
+  const uint32_t n = *n_ptr;
+  return n < n_max ? n : n_max;
+}
 __global__ __launch_bounds__(kWg) void request_first_kernel(RequestSource req, uint32_t n_max, const uint32_t *n_ptr,
                                                             unsigned long long *first, uint32_t gen, uint32_t n_blocks,
                                                             uint32_t *flags) {
-  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-  if (i >= n_max) return;
-  const bool takes_part = i < *n_ptr && request_inserts(req, i, n_blocks);
-  if (takes_part) atomicMax(&first[request_key(req, i).address], first_key(gen, i));
-  flags[i] = takes_part ? 1u : 0u;
+  const uint32_t n = stored_count(n_ptr, n_max);
+  for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += gridDim.x * kWg) {
+    const bool takes_part = request_inserts(req, i, n_blocks);
+    if (takes_part) atomicMax(&first[request_key(req, i).address], first_key(gen, i));
+    flags[i] = takes_part ? 1u : 0u;
+  }
 }
 
 // 1 when request i is the first for its address and its block can take a child level (the
 // builder refines exactly the blocks below max_depth that can have children); 0 beyond the
 // stored requests.
 // (after request_first_kernel) flags[i] stays 1 only for a taking-part request
-__global__ __launch_bounds__(kWg) void request_flag_kernel(RequestSource req, uint32_t n_max,
+__global__ __launch_bounds__(kWg) void request_flag_kernel(RequestSource req, uint32_t n_max, const uint32_t *n_ptr,
                                                            const unsigned long long *first, uint32_t gen,
                                                            const uint4 *blocks, uint32_t max_depth, uint32_t *flags) {
-  const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-  if (i >= n_max || flags[i] == 0u) return;
-  const RequestKey k = request_key(req, i);
-  const uint32_t a = k.address, depth = k.depth;
-  bool ok = first[a] == first_key(gen, i) && depth + 1u <= max_depth && depth >= 1;
-  if (ok) {
-    const uint4 blk = blocks[a];
-    // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
-    const uint32_t chunk = blk.y & 0xffffu;
-    ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
+  const uint32_t n = stored_count(n_ptr, n_max);
+  for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < n; i += gridDim.x * kWg) {
+    if (flags[i] == 0u) continue;
+    const RequestKey k = request_key(req, i);
+    const uint32_t a = k.address, depth = k.depth;
+    bool ok = first[a] == first_key(gen, i) && depth + 1u <= max_depth && depth >= 1;
+    if (ok) {
+      const uint4 blk = blocks[a];
+      // childless, and CanHaveChildren (TreeBuilder.cs:291-302): an octant with air and geometry
+      const uint32_t chunk = blk.y & 0xffffu;
+      ok = blk.x == kU32Max && ((chunk & (chunk >> 1)) & 0x5555u) != 0u;
+    }
+    if (!ok) flags[i] = 0u;
   }
-  if (!ok) flags[i] = 0u;
 }
 
 // Request list[k] becomes graft record k and brick blocks [8k, 8k+8); eight threads a brick.
@@ -335,9 +344,21 @@ __global__ __launch_bounds__(kWg) void select_scatter_kernel(const svo_usage *u,
     if (is[j]) list[r++] = g0 + j;
 }
 
-// Exclusive ranks of the flagged entries (flags 0/1), chunk by chunk like the selection.
-__global__ __launch_bounds__(kWg) void flag_count_kernel(const uint32_t *flags, uint32_t n, uint32_t *counts) {
+// Exclusive ranks of the flagged entries (flags 0/1), chunk by chunk like the selection.  With
+// n_dev the flags are defined only below min(n, *n_dev): later chunks count zero.
+__device__ __forceinline__ uint32_t flagged_count(uint32_t n, const uint32_t *n_dev) {
+  if (!n_dev) return n;
+  const uint32_t d = *n_dev;
+  return d < n ? d : n;
+}
+__global__ __launch_bounds__(kWg) void flag_count_kernel(const uint32_t *flags, uint32_t n, const uint32_t *n_dev,
+                                                         uint32_t *counts) {
   __shared__ uint32_t s_waves[kWg / 64];
+  n = flagged_count(n, n_dev);
+  if (blockIdx.x * kSelChunk >= n) {  // uniform over the work-group
+    if (threadIdx.x == 0) counts[blockIdx.x] = 0u;
+    return;
+  }
   const uint32_t i0 = blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
   uint32_t c = 0;
   for (uint32_t j = 0; j < kSelPer; j++) c += i0 + j < n ? flags[i0 + j] : 0u;
@@ -346,10 +367,14 @@ __global__ __launch_bounds__(kWg) void flag_count_kernel(const uint32_t *flags, 
   if (threadIdx.x == 0) counts[blockIdx.x] = all;
 }
 
-// rank[i] = flagged entries before i; with a list, flagged i of rank r < limit is list[r].
-__global__ __launch_bounds__(kWg) void flag_rank_kernel(const uint32_t *flags, uint32_t n, const uint32_t *offsets,
-                                                        uint32_t *rank, uint32_t *list, uint32_t limit) {
+// rank[i] = flagged entries before i (when rank is given); with a list, flagged i of rank
+// r < limit is list[r].  With n_dev, entries from min(n, *n_dev) on are not flagged.
+__global__ __launch_bounds__(kWg) void flag_rank_kernel(const uint32_t *flags, uint32_t n, const uint32_t *n_dev,
+                                                        const uint32_t *offsets, uint32_t *rank, uint32_t *list,
+                                                        uint32_t limit) {
   __shared__ uint32_t s_waves[kWg / 64];
+  n = flagged_count(n, n_dev);
+  if (blockIdx.x * kSelChunk >= n) return;  // uniform over the work-group
   const uint32_t i0 = blockIdx.x * kSelChunk + threadIdx.x * kSelPer;
   uint32_t f[kSelPer], c = 0;
   for (uint32_t j = 0; j < kSelPer; j++) {
@@ -359,7 +384,7 @@ __global__ __launch_bounds__(kWg) void flag_rank_kernel(const uint32_t *flags, u
   uint32_t all;
   uint32_t r = offsets[blockIdx.x] + wg_scan<kWg>(c, s_waves, &all);
   for (uint32_t j = 0; j < kSelPer; j++) {
-    if (i0 + j < n) rank[i0 + j] = r;
+    if (rank && i0 + j < n) rank[i0 + j] = r;
     if (list && f[j] && r < limit) list[r] = i0 + j;
     r += f[j];
   }
@@ -693,11 +718,11 @@ int launch_request_unique(const RequestSource &req, uint32_t n_max, const uint32
                           uint32_t gen, const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s) {
   if (!n_max) return 0;
-  hipLaunchKernelGGL(request_first_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, n, first, gen,
-                     n_blocks, flags);
+  const uint32_t grid = std::min((n_max + kWg - 1) / kWg, kRequestGrid);
+  hipLaunchKernelGGL(request_first_kernel, dim3(grid), dim3(kWg), 0, s, req, n_max, n, first, gen, n_blocks, flags);
   if (rc_of()) return -1;
-  hipLaunchKernelGGL(request_flag_kernel, dim3((n_max + kWg - 1) / kWg), dim3(kWg), 0, s, req, n_max, first, gen,
-                     blocks, max_depth, flags);
+  hipLaunchKernelGGL(request_flag_kernel, dim3(grid), dim3(kWg), 0, s, req, n_max, n, first, gen, blocks, max_depth,
+                     flags);
   return rc_of();
 }
 
@@ -710,13 +735,13 @@ int launch_bricks(const TerrainView &t, const RequestSource &req, const uint32_t
 }
 
 int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total,
-                uint32_t *span_sums, hipStream_t s, uint32_t *list, uint32_t limit) {
+                uint32_t *span_sums, hipStream_t s, uint32_t *list, uint32_t limit, const uint32_t *n_dev) {
   const uint32_t chunks = (n + kSelChunk - 1) / kSelChunk;
   if (!chunks) return 0;
-  hipLaunchKernelGGL(flag_count_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts);
+  hipLaunchKernelGGL(flag_count_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, n_dev, counts);
   if (rc_of()) return -1;
   if (launch_scan_u32(counts, counts, chunks, total, span_sums, s)) return -1;
-  hipLaunchKernelGGL(flag_rank_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, counts, rank, list, limit);
+  hipLaunchKernelGGL(flag_rank_kernel, dim3(chunks), dim3(kWg), 0, s, flags, n, n_dev, counts, rank, list, limit);
   return rc_of();
 }
 

@@ -42,20 +42,22 @@ struct RequestSource {
   const uint32_t *cones;  // null: records in request order
   const uint2 *keys;
 };
-// flags[i] (i < n_max) = 1 for the first request of each block that can take a child level
-// among the *n stored requests; first: u64 per block of the pool, entries of generations
-// below gen are ignored (zero-filled when allocated)
+// flags[i] (i < min(*n, n_max), not written beyond) = 1 for the first request of each block
+// that can take a child level among the *n stored requests; first: u64 per block of the
+// pool, entries of generations below gen are ignored (zero-filled when allocated)
 int launch_request_unique(const RequestSource &req, uint32_t n_max, const uint32_t *n, unsigned long long *first,
                           uint32_t gen, const uint4 *blocks, uint32_t n_blocks, uint32_t max_depth, uint32_t *flags,
                           hipStream_t s);
 // bricks of the first min(*unique, nb_max) listed requests
 int launch_bricks(const TerrainView &t, const RequestSource &req, const uint32_t *list, uint32_t nb_max,
                   const uint32_t *unique, svo_grafting *records, uint4 *bricks, hipStream_t s);
-// rank[i] = number of flagged entries before i; counts holds one entry per 1024 flags.  With
-// a list, the flagged entries of rank below limit are listed in rank order.  span_sums:
-// scratch of the scan over the counts (scan_span_sums_bytes of the count of chunks).
+// rank[i] = number of flagged entries before i (rank may be null); counts holds one entry per
+// 1024 flags.  With a list, the flagged entries of rank below limit are listed in rank order.
+// span_sums: scratch of the scan over the counts (scan_span_sums_bytes of the count of
+// chunks).  n_dev: the flags are defined only below min(n, *n_dev) (a device-side count).
 int launch_rank(const uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t *rank, uint32_t *total,
-                uint32_t *span_sums, hipStream_t s, uint32_t *list = nullptr, uint32_t limit = 0);
+                uint32_t *span_sums, hipStream_t s, uint32_t *list = nullptr, uint32_t limit = 0,
+                const uint32_t *n_dev = nullptr);
 // Groups of [lo, hi) that satisfy a predicate, in ascending order: per-chunk counts, then the
 // scatter of the list given the scanned counts.
 uint32_t select_chunks(uint32_t lo, uint32_t hi);
