@@ -326,8 +326,10 @@ enum {
                                         buffer's size when it is bound) with an 8-byte key each;
                                         ChildRequests is not written.  For svo_trace_frame and
                                         svo_trace_batch (SVO_ERR_NOT_SUPPORTED elsewhere); needs
-                                        ChildRequestID.  A later direct frame without the flag drops
-                                        the kept requests. */
+                                        ChildRequestID.  Any later frame that does not keep its own
+                                        drops them: a direct frame without the flag, a frame graph
+                                        launch, svo_trace_frame_static, or Init, Trace, RunBaseTrace
+                                        or RunBlockTrace through svo_enqueue. */
 };
 
 /* One primary frame (MainWindow.RunWavefront, MainWindow.cs:356-426: init, the base/block
@@ -633,7 +635,7 @@ typedef struct {
 /* One streaming step after a traced frame (the WorldManager.UpdateGrafting / RunGraft role,
  * WorldManager.cs:84-103, MainWindow.cs:307-337, fed by requestChild, wavefront.cl:362-375):
  *  1. the stored child requests (ChildRequests, count in ChildRequestID; or, when the context's
- *     last direct frame was traced with SVO_FRAME_KEEP_REQUESTS, the requests it kept, under the
+ *     last frame was a direct frame traced with SVO_FRAME_KEEP_REQUESTS, the requests it kept, under the
  *     same limits: the same result without the 40-byte copies) are read in order;
  *     among those asking for a level (TreeSize >= 1: the cone needs detail below the block),
  *     the first request of every block address whose block is childless, at depth 1 <=

@@ -103,6 +103,8 @@ extern "C" int svo_trace_frame_static(svo_context *ctx, const svo_trace_input *i
                                       " bytes and more) exceeds the static maximum of " +
                                       std::to_string(kStaticScratchBytes) + " bytes");
   if (rc) return rc;
+  // this frame writes ChildRequests: earlier kept requests are no longer the last frame's
+  ctx->kept.valid = false;
   // the camera record rides in the first launch's argument; nothing else is copied
   if (launch_begin(plan, &host_cam, 1, ctx->stream) != 0)
     return hip_fail(hipGetLastError(), "svo_trace_frame_static: frame begin launch");

@@ -180,6 +180,8 @@ extern "C" int svo_frame_graph_launch(svo_context *ctx, svo_frame_graph *g, cons
   } else {
     fill_camera_poses(inputs, ap, g->n_cams, g->host);
   }
+  // the replayed frames write ChildRequests: earlier kept requests are no longer the last frame's
+  ctx->kept.valid = false;
   hipError_t e = hipGraphLaunch(g->exec, ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(g->done, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");

@@ -144,9 +144,11 @@ struct Context {
   uint32_t *stream_host = nullptr;
   // generation of the streaming update's first-request table (0: never used)
   uint32_t stream_gen = 0;
-  // The child requests of the last direct frame, when it kept them (SVO_FRAME_KEEP_REQUESTS):
-  // the first `capacity` requesting cone ids in ascending order, a key per requesting cone
-  // and the full records by cone id, all in frame_scratch.
+  // The child requests of the context's last frame, when it was a direct frame that kept them
+  // (SVO_FRAME_KEEP_REQUESTS): the first `capacity` requesting cone ids in ascending order, a
+  // key per requesting cone and the full records by cone id, all in frame_scratch.  Every
+  // other frame (direct without the flag, a graph replay, the allocation-free frame, a trace
+  // kernel through svo_enqueue) clears `valid`: its requests went to ChildRequests.
   struct KeptRequests {
     bool valid = false;
     const uint32_t *cones = nullptr;

@@ -199,14 +199,18 @@ def test_stream_request_capacity(svo, orc, gpu, keep, unbind):
     assert any(h["requests"] > 45 for h in history) and sum(h["bricks"] for h in history) > 50
 
 
-def test_kept_requests_rules(svo, orc, gpu):
+@pytest.mark.parametrize("dropper", ["direct", "static", "graph"])
+def test_kept_requests_rules(svo, orc, gpu, dropper):
     """SVO_FRAME_KEEP_REQUESTS is refused by frame graphs and the allocation-free frame, and a
-    direct frame traced without it drops the kept requests: the update then reads ChildRequests."""
+    later frame that does not keep its requests (a direct frame without the flag, the
+    allocation-free frame, a graph replay) drops the kept ones: the update then reads
+    ChildRequests."""
     geom = terrain_geometry(svo, 64)
     B, D = 2, 7
     W, H = 48, 32
     st = static_tree(svo, geom, B, D)
     gs = GpuStream(svo, gpu, geom, B, 8 ** 3 + 600, W, H)
+    graph = None
     try:
         cam_a = svo.scenes.config3_camera(0, W, H, frames=10, base_depth=B)
         cam_b = svo.scenes.config3_camera(6, W, H, frames=10, base_depth=B)
@@ -217,11 +221,25 @@ def test_kept_requests_rules(svo, orc, gpu):
             gpu.TraceFrameStatic(cam_a, flags=svo.FRAME_KEEP_REQUESTS)
         bases, blocks, usages = gs.pool()
         pool = orc.TreePool(bases, blocks, usages)
-        gs.trace(cam_a, 1, keep=True)
-        got = gs.trace(cam_b, 1)  # same tick, another view, requests to ChildRequests
         cam_b.tick = 1
+        cam_b.max_child_request_id = W * H
+        if dropper == "graph":  # captured before the kept frame: capturing runs nothing
+            graph = gpu.CaptureFrame([cam_b], flags=svo.FRAME_STATE)
+        gs.trace(cam_a, 1, keep=True)
+        # same tick, another view, requests to ChildRequests
+        if dropper == "direct":
+            got_requests = gs.trace(cam_b, 1)["requests"]
+        else:
+            if dropper == "static":
+                gpu.TraceFrameStatic(cam_b, flags=svo.FRAME_STATE)
+            else:
+                graph.Launch([cam_b])
+            gpu.Wait()
+            N = svo.BufferName
+            count = int(gpu.ReadBuffer(N.ChildRequestID, np.uint32, 1)[0])
+            got_requests = gpu.ReadBuffer(N.ChildRequests, svo.CHILD_REQUEST_DTYPE, min(count, W * H))
         ref = orc.trace_frame(cam_b, pool.bases, pool.blocks, pool.usages, request_capacity=W * H, threads=8)
-        _eq(got["requests"], ref["requests"], "requests")
+        _eq(got_requests, ref["requests"], "requests")
         upd = gs.update(1, D, max_bricks=50, prune_age=2, gc_free_percent=0)
         exp = orc.stream_update(pool, ref["requests"], ref["request_count"], W * H, B, D, 1,
                                 lambda sel: static_children(st.blocks, B, sel), max_bricks=50, prune_age=2,
@@ -232,6 +250,8 @@ def test_kept_requests_rules(svo, orc, gpu):
         _eq(gblocks, pool.blocks, "blocks")
         _eq(gusages, pool.usages, "usages")
     finally:
+        if graph is not None:
+            graph.Dispose()
         gs.close()
 
 
